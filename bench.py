@@ -1,0 +1,212 @@
+"""Forward+backward rays/s of the plain renderer on one MI355X.
+
+Workload (the plain full-frame configuration): 128^3 quadric grid of a
+torus (R=1, r=0.35 world units, + N(0, 0.005) noise), SH degree 2
+(32 appearance channels, coefficients N(0, 0.1)), 100 camera poses on a
+radius-4 sphere, FOV 40 deg, 800x800, 192 uniform samples per ray, logistic
+coefficient 5.  One step = one pose: ray generation + forward + MSE loss
+against a random target + backward into zeroed gradient grids, over all
+640,000 pixel rays.  Synthetic data, random-init grids.
+
+Prints ONE JSON line (rank 0).  Extra fields: the op-surface path's rays/s,
+the proposal-guided figure when its network is available, the dominant
+kernel's roofline and the CPU oracle baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("fwd+bwd rays/s at 128³ grid 800×800, plain vs proposal at "
+          "≤0.1 dB PSNR loss")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--op-steps", type=int, default=2, help="steps of the op-surface path (0: skip)")
+    ap.add_argument("--cpu-rays", type=int, default=16384,
+                    help="rays in the CPU oracle sample (0: skip the CPU baseline)")
+    ap.add_argument("--scatter", type=int, default=0, help="fused scatter mode (0 tile, 1 atomic)")
+    return ap.parse_args()
+
+
+def algorithmic_bytes(R, C, P):
+    """Compulsory HBM bytes of one fused backward launch over P rays: read the
+    per-ray colour, depth and their gradients (32 B), read both grids once,
+    read-modify-write both gradient grids once."""
+    G = 28 * R ** 3 + 4 * C * (R + 1) ** 3
+    return 32 * P + 3 * G
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the path "
+                                   "shards by pose with one gradient all-reduce per step, "
+                                   "described in DESIGN.md)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from qsdvr_amd import fused, scenes
+    from qsdvr_amd import render as hr
+    from qsdvr_amd import qsdvr
+    from qsdvr_amd.camera import Camera
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = "cuda"
+    w = scenes.WORKLOADS["plain"]
+    t0 = time.time()
+    grid, data, gen = scenes.build_grids(w, seed=0, device=dev)
+    cams = scenes.orbit_cameras(w.n_poses, w.width, w.height)
+    cam_buf = fused.camera_buffer(cams, dev)
+    P = w.width * w.height
+    targets = torch.rand((w.n_poses, P, 3), generator=torch.Generator().manual_seed(1)).to(dev)
+    log(f"setup {time.time() - t0:.1f}s")
+    G = grid.requires_grad_(True)
+    D = data.requires_grad_(True)
+    CAM_BYTES = 80
+
+    def fused_step(i):
+        p = i % w.n_poses
+        G.grad = None
+        D.grad = None
+        cb = cam_buf[p * CAM_BYTES:(p + 1) * CAM_BYTES]
+        c, _ = fused.FusedRender.apply(G, D, cb, 1, w.width, w.height, None, w.interval, w.s,
+                                       args.scatter)
+        loss = ((c - targets[p]) ** 2).mean()
+        loss.backward()
+        return loss
+
+    for i in range(args.warmup):
+        fused_step(i)
+    torch.cuda.synchronize()
+    fused.TIMERS = []
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for i in range(args.steps):
+        fused_step(args.warmup + i)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    timers, fused.TIMERS = fused.TIMERS, None
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+    rays_s = P / (ms_step / 1e3)
+    kern = {}
+    for name, a, b in timers:
+        kern.setdefault(name, []).append(a.elapsed_time(b))
+    kern_ms = {k: statistics.mean(v) for k, v in kern.items()}
+    log(f"fused: {ms_step:.3f} ms/step, {rays_s:.4g} rays/s (wall {wall:.2f}s); kernels {kern_ms}")
+
+    # ------------------------------------------------ op-surface path ----
+    op_rays = None
+    if args.op_steps > 0:
+        G2 = grid.detach().clone().requires_grad_(True)
+        D2 = data.detach().clone().requires_grad_(True)
+        cam_objs = []
+        for p in range(args.op_steps + 1):
+            a = qsdvr.CameraArgs.from_camera(cams[p])
+            c = Camera(a, w.interval, w.R)
+            c.GenerateRayPoints(None)
+            cam_objs.append(c)
+
+        def op_step(i):
+            G2.grad = None
+            D2.grad = None
+            inp = cam_objs[i].GetRayPoints()
+            out = hr.render_lists(G2, D2, inp, w.s)
+            ((out - targets[i]) ** 2).mean().backward()
+
+        op_step(0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for i in range(args.op_steps):
+            op_step(1 + i)
+        e1.record()
+        torch.cuda.synchronize()
+        op_ms = e0.elapsed_time(e1) / args.op_steps
+        op_rays = P / (op_ms / 1e3)
+        log(f"op-surface: {op_ms:.3f} ms/step, {op_rays:.4g} rays/s")
+        del G2, D2, cam_objs
+        torch.cuda.empty_cache()
+
+    # ------------------------------------------------------- roofline ----
+    dom = max(kern_ms, key=kern_ms.get)
+    byts = algorithmic_bytes(w.R, 32, P) if dom == "fused_backward" else (
+        28 * w.R ** 3 + 4 * 32 * (w.R + 1) ** 3 + 16 * P)
+    achieved = byts / (kern_ms[dom] / 1e3) / 1e9
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": None, "algorithmic_bytes_per_launch": byts,
+                "avg_launch_ms": round(kern_ms[dom], 4)}
+
+    # --------------------------------------------------- CPU baseline ----
+    cpu = None
+    if args.cpu_rays > 0:
+        cpu = cpu_baseline(w, cams[0], grid.detach().cpu(), data.detach().cpu(),
+                           targets[0].cpu(), args.cpu_rays)
+
+    out = {"metric": METRIC, "value": round(rays_s, 1), "unit": "rays/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (torus SDF quadric grid, random appearance, random targets)",
+           "config": {"workload": w.name, "grid": w.R, "sh_degree": w.sh_degree,
+                      "width": w.width, "height": w.height, "samples": w.n_samples,
+                      "poses": w.n_poses, "logistic_coef": w.s, "path": "fused plain",
+                      "scatter_mode": args.scatter},
+           "op_surface_rays_per_s": round(op_rays, 1) if op_rays else None,
+           "kernel_ms": {k: round(v, 4) for k, v in kern_ms.items()},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(w, cam, grid, data, target, n_rays):
+    """The CPU oracle (render_torch restated; torch CPU, float32,
+    vectorised) timed forward+backward on a fixed subset of one pose's rays."""
+    import numpy as np
+    import torch
+
+    from oracle import raygen
+    from oracle import render as orc
+
+    interval = np.float32(w.interval)
+    k = raygen.ray_cut_kernel(cam.as_dict(), interval)
+    P = k["count"].shape[0]
+    sel = np.linspace(0, P - 1, n_rays).astype(np.int64)
+    mask = np.zeros(P, dtype=bool)
+    mask[sel] = True
+    smp = raygen.ray_samples(cam.as_dict(), interval, w.R, mask=mask)
+    tgt = target[torch.as_tensor(sel)]
+    threads = torch.get_num_threads()
+    G = grid.clone().requires_grad_(True)
+    D = data.clone().requires_grad_(True)
+    t0 = time.perf_counter()
+    col = orc.render(G, D, smp, w.R, w.s)
+    orc.mse_loss(col, tgt).backward()
+    dt = time.perf_counter() - t0
+    return {"value": round(n_rays / dt, 1), "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"{n_rays} rays (every {P // n_rays}th pixel of pose 0), "
+                      f"{smp['renderCount']} render samples, fwd+bwd, float32",
+            "seconds": round(dt, 2)}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,257 @@
+/*
+ * qsdvr_hip.h -- C ABI of the MI355X (gfx950) QSDVR render hot path.
+ *
+ * One entry point per function the reference binds in kernel/qsdvr.cpp:5-50,
+ * plus the fused renderer and the proposal network, which have no reference
+ * counterpart.  Conventions shared by every entry point:
+ *
+ *  - Plain pointers and sizes only.  Pointers are device pointers unless the
+ *    argument says "host".  Sizes are 64-bit element counts.
+ *  - The caller owns all memory.  Entry points never allocate; scratch comes
+ *    in through a (workspace, workspace_bytes) pair sized by the matching
+ *    *_workspace_size() call.
+ *  - Forward ops overwrite their outputs.  The three backward ops the
+ *    reference documents as accumulating (GridInterpolationBackward,
+ *    SampleSDFBackward, SampleNormalBackward) add into a caller-zeroed grid;
+ *    the others overwrite.
+ *  - Work is enqueued on `stream` (a hipStream_t; NULL = the null stream).
+ *    Nothing synchronises the host except qsdvr_ray_cut (it must return the
+ *    two sample totals) and qsdvr_layer_to_grid_backward with host outputs.
+ *  - Every call returns QSDVR_OK (0) or a negative status; the message of
+ *    the calling thread's last failure is available from qsdvr_last_error().
+ *  - Sample / cell indices read from caller buffers are range-checked in the
+ *    kernels: an out-of-range index yields NaN outputs (forward) or no
+ *    scatter (backward) instead of an out-of-bounds access.
+ *  - No global mutable state besides the thread-local error message;
+ *    calls on different streams may run concurrently.
+ */
+#ifndef QSDVR_HIP_H
+#define QSDVR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSDVR_ABI_VERSION 1
+
+enum {
+    QSDVR_OK = 0,
+    QSDVR_EINVAL = -1,    /* bad argument (null pointer, size, shape)    */
+    QSDVR_EHIP = -2,      /* a HIP runtime call or launch failed          */
+    QSDVR_ERANGE = -3,    /* a total or size exceeds what the format holds */
+    QSDVR_EWORKSPACE = -4 /* workspace missing or too small               */
+};
+
+/* Pinhole camera, the fields of CameraArgs (include/camera.h:4-17):
+ * row-major 3x3 camera-to-world rotation, translation, focal lengths,
+ * principal point (cx = -W/2, cy = -H/2 in the reference's datasets) and
+ * near/far camera-space depths. */
+typedef struct qsdvr_camera {
+    uint32_t width;
+    uint32_t height;
+    float rotation[9];
+    float translation[3];
+    float fx, fy, cx, cy;
+    float near_depth, far_depth;
+} qsdvr_camera;
+
+typedef void *qsdvr_stream; /* hipStream_t */
+
+int qsdvr_abi_version(void);
+const char *qsdvr_last_error(void);
+
+/* ------------------------------------------------------------ camera --- */
+
+/* Replaces RayCut (kernel/camera.cu:98-130, bound at kernel/qsdvr.cpp:32,
+ * prototype include/camera.h:57).  Per pixel: ray direction, first in-cube
+ * step (origin) and the count of steps strictly inside +-0.9999; then, over
+ * the masked pixels in order, sdfCount (<2 -> 0), renderCount =
+ * max(sdfCount-1, 0) and both exclusive offsets.  ray_list is int32 [P,4]
+ * {sdfCount, sdfOffset, renderCount, renderOffset}; unmasked rows receive
+ * only column 0 (the raw count).  mask_host: host bool/uint8 [P] or NULL
+ * (all pixels).  Writes the two totals to *sdf_total / *render_total (host)
+ * after synchronising the stream. */
+size_t qsdvr_ray_cut_workspace_size(int64_t pixel_count);
+int qsdvr_ray_cut(const qsdvr_camera *cam, const uint8_t *mask_host,
+                  int32_t *ray_list, float *origin, float *dir,
+                  float interval, int64_t pixel_count,
+                  void *workspace, size_t workspace_bytes,
+                  int64_t *sdf_total, int64_t *render_total,
+                  qsdvr_stream stream);
+
+/* Replaces GenerateRayPoints (kernel/camera.cu:132-146, bound at
+ * kernel/qsdvr.cpp:34, prototype include/camera.h:59).  For each of the
+ * ray_count rows of ray_list: sdf samples ori + (dir*j)*interval and render
+ * samples ori + dir*((j+0.5)*interval), mapped to the cell index
+ * x + yR + zR^2 and local point in [-1,1)^3 with R = base_reso.  Bit-exact
+ * float32 (no contraction).  sdf_total / render_total are the output
+ * lengths (the RayCut totals) and bound every write. */
+int qsdvr_generate_ray_points(float *sdf_point, int32_t *sdf_index,
+                              float *render_point, int32_t *render_index,
+                              float *view_dir, const int32_t *ray_list,
+                              const float *origin, const float *dir,
+                              int64_t ray_count, int64_t sdf_total,
+                              int64_t render_total, float interval,
+                              int32_t base_reso, qsdvr_stream stream);
+
+/* ----------------------------------------------------------- surface --- */
+
+/* Replaces LayerToGridForward (kernel/surface.cu:148-229, bound at
+ * kernel/qsdvr.cpp:12, prototype include/surface.h:4).  Layers x,y,z [R]
+ * and offset [4] are host pointers when layers_on_host != 0 (as in the
+ * reference), device pointers otherwise.  grid_out: [R,R,R,7]. */
+size_t qsdvr_layer_to_grid_workspace_size(int32_t reso);
+int qsdvr_layer_to_grid_forward(float *grid_out, const float *x_layer,
+                                const float *y_layer, const float *z_layer,
+                                const float *offset, int32_t reso,
+                                int layers_on_host, void *workspace,
+                                size_t workspace_bytes, qsdvr_stream stream);
+
+/* Replaces LayerToGridBackward (kernel/surface.cu:231-341, bound at
+ * kernel/qsdvr.cpp:14, prototype include/surface.h:6).  grad_grid
+ * [R,R,R,7] (device) -> gradients of x,y,z [R] and offset [4] (overwrite;
+ * host pointers when outputs_on_host != 0, then the stream is synchronised).
+ * offset[3] receives the full sum of the G gradients. Deterministic. */
+int qsdvr_layer_to_grid_backward(const float *grad_grid, float *gx, float *gy,
+                                 float *gz, float *goffset, int32_t reso,
+                                 int outputs_on_host, void *workspace,
+                                 size_t workspace_bytes, qsdvr_stream stream);
+
+/* Replaces SampleSDFForward (kernel/surface.cu:343-350, bound at
+ * kernel/qsdvr.cpp:16, prototype include/surface.h:8):
+ * out[i] = f(p)/||grad f(p)||/reso for the quadric of cell index[i]. */
+int qsdvr_sample_sdf_forward(float *out, const float *sdf_grid,
+                             const float *point, const int32_t *index,
+                             int64_t n, int64_t cell_count, float reso,
+                             qsdvr_stream stream);
+
+/* Replaces SampleSDFBackward (kernel/surface.cu:352-359, bound at
+ * kernel/qsdvr.cpp:18, prototype include/surface.h:10): accumulates
+ * d sdf / d(A..G) * grad_out into the caller-zeroed grad_grid. */
+int qsdvr_sample_sdf_backward(const float *grad_out, const float *sdf_grid,
+                              const float *point, const int32_t *index,
+                              float *grad_grid, int64_t n, int64_t cell_count,
+                              float reso, qsdvr_stream stream);
+
+/* Replaces SampleNormalForward (kernel/surface.cu:361-367, bound at
+ * kernel/qsdvr.cpp:20, prototype include/surface.h:12):
+ * out = grad f / max(||grad f||, 1e-12), [n,3]. */
+int qsdvr_sample_normal_forward(float *out, const float *sdf_grid,
+                                const float *point, const int32_t *index,
+                                int64_t n, int64_t cell_count,
+                                qsdvr_stream stream);
+
+/* Replaces SampleNormalBackward (kernel/surface.cu:369-375, bound at
+ * kernel/qsdvr.cpp:22, prototype include/surface.h:14): accumulates into
+ * the caller-zeroed grad_grid (A..F; G receives nothing). */
+int qsdvr_sample_normal_backward(const float *grad_out, const float *sdf_grid,
+                                 const float *point, const int32_t *index,
+                                 float *grad_grid, int64_t n,
+                                 int64_t cell_count, qsdvr_stream stream);
+
+/* ------------------------------------------------------------ render --- */
+
+/* Replaces GridInterpolationForward (kernel/render.cu:317-323, bound at
+ * kernel/qsdvr.cpp:8, prototype include/render.h:4).  vertex_reso is what
+ * the reference passes (dataGrid.size(0) = R+1); rows are addressed as in
+ * models/render_torch.py:128-143: base row = cell index, neighbours +1, +R,
+ * +R^2, weights = raw local point.  data_grid [(R+1)^3, channels],
+ * out [n, channels]. */
+int qsdvr_grid_interpolation_forward(const float *data_grid, const float *point,
+                                     const int32_t *index, float *out,
+                                     int64_t n, int32_t vertex_reso,
+                                     int32_t channels, qsdvr_stream stream);
+
+/* Replaces GridInterpolationBackward (kernel/render.cu:325-331, bound at
+ * kernel/qsdvr.cpp:10, prototype include/render.h:6).  Same argument roles
+ * as the reference: grad_grid is the caller-zeroed accumulator,
+ * grad_out [n, channels] the upstream gradient. */
+int qsdvr_grid_interpolation_backward(float *grad_grid, const float *point,
+                                      const int32_t *index,
+                                      const float *grad_out, int64_t n,
+                                      int32_t vertex_reso, int32_t channels,
+                                      qsdvr_stream stream);
+
+/* Replaces ShaderForward (kernel/render.cu:333-339, bound at
+ * kernel/qsdvr.cpp:24, prototype include/render.h:8).  rgb = ao*(diffuse*
+ * (1-metallic) + SH(reflect(view, n))*metallic); specular = SH(...).
+ * channels 32 (SH degree <= 2, coefficient k colour c at 3k+c) or 53
+ * (degree 3). */
+int qsdvr_shader_forward(float *out, const float *normal, const float *view_dir,
+                         const float *data, float *specular, int64_t n,
+                         int32_t channels, qsdvr_stream stream);
+
+/* Replaces ShaderBackward (kernel/render.cu:341-347, bound at
+ * kernel/qsdvr.cpp:26, prototype include/render.h:10).  Overwrites
+ * normal_grad [n,3] and data_grad [n,channels]. */
+int qsdvr_shader_backward(const float *grad_out, const float *normal,
+                          const float *view_dir, const float *data,
+                          const float *specular, float *normal_grad,
+                          float *data_grad, int64_t n, int32_t channels,
+                          qsdvr_stream stream);
+
+/* Replaces RayAggregateForward (kernel/render.cu:349-355, bound at
+ * kernel/qsdvr.cpp:28, prototype include/render.h:12).  NeuS logistic
+ * compositing per ray of ray_list; writes colour [rays,3] and the per
+ * render-sample alpha and transmittance T. */
+int qsdvr_ray_aggregate_forward(float *out, const float *rgb, const float *sdf,
+                                const int32_t *ray_list, float *alpha, float *T,
+                                int64_t ray_count, int64_t sdf_total,
+                                int64_t render_total, float logistic_coef,
+                                qsdvr_stream stream);
+
+/* Replaces RayAggregateBackward (kernel/render.cu:357-363, bound at
+ * kernel/qsdvr.cpp:30, prototype include/render.h:14).  Overwrites
+ * rgb_grad [S_r,3] and sdf_grad [S_sdf] on the rows of non-empty rays; the
+ * alpha adjoint uses the division-free suffix recurrence. */
+int qsdvr_ray_aggregate_backward(const float *grad_out, const float *rgb,
+                                 const float *sdf, const int32_t *ray_list,
+                                 const float *alpha, const float *T,
+                                 float *rgb_grad, float *sdf_grad,
+                                 int64_t ray_count, int64_t sdf_total,
+                                 int64_t render_total, float logistic_coef,
+                                 qsdvr_stream stream);
+
+/* ------------------------------------------------------------- fused --- */
+
+/* No reference counterpart.  The whole plain render of kernel/camera.cu
+ * ray generation + SampleSDF + SampleNormal + GridInterpolation + Shader +
+ * RayAggregate in one pass.  Rays: when ray_pixel == NULL, every pixel of
+ * the n_cams cameras (all width x height), in row-major order per camera;
+ * otherwise ray_pixel[i] = cam * W*H + pixel for n_rays rays.  Outputs per
+ * ray: color [n_rays,3], depth [n_rays] (sum of T_i alpha_i times the
+ * distance from the camera centre to render sample i; nullable).
+ * cams is a DEVICE array of n_cams qsdvr_camera; a camera whose size differs
+ * from width x height renders black.  Sample positions and cell indices are
+ * those of RayCut + GenerateRayPoints, bit for bit. */
+int qsdvr_fused_forward(const qsdvr_camera *cams, int32_t n_cams,
+                        int32_t width, int32_t height,
+                        const int32_t *ray_pixel, int64_t n_rays,
+                        const float *sdf_grid, const float *data_grid,
+                        int32_t reso, int32_t channels, float interval,
+                        float logistic_coef, float *color, float *depth,
+                        qsdvr_stream stream);
+
+/* Adjoint of qsdvr_fused_forward.  Recomputes alpha and T front to back
+ * from the saved per-ray colour/depth (no per-sample storage).  Accumulates
+ * into the caller-zeroed sdf_grid_grad [R^3,7] and data_grid_grad
+ * [(R+1)^3, channels].  grad_depth may be NULL.  scatter_mode:
+ * 0 = tile-reduced atomics, 1 = plain per-sample atomics. */
+int qsdvr_fused_backward(const qsdvr_camera *cams, int32_t n_cams,
+                         int32_t width, int32_t height,
+                         const int32_t *ray_pixel, int64_t n_rays,
+                         const float *sdf_grid, const float *data_grid,
+                         int32_t reso, int32_t channels, float interval,
+                         float logistic_coef, const float *color,
+                         const float *depth, const float *grad_color,
+                         const float *grad_depth, float *sdf_grid_grad,
+                         float *data_grid_grad, int32_t scatter_mode,
+                         qsdvr_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSDVR_HIP_H */

@@ -1,0 +1,373 @@
+// Fused plain renderer for gfx950: ray generation + SDF + normal +
+// trilinear appearance + SH shading + NeuS compositing in one pass, and its
+// recompute adjoint.  No reference counterpart as one kernel; it computes
+// exactly the composition of kernel/camera.cu:12-96 (bit-exact sample
+// positions and cell indices) with the math of models/render_torch.py:77-167.
+//
+// Work mapping: one lane per ray; a wavefront owns an 8x8 pixel tile, so
+// the 64 rays of a wave march side by side through the same few cells and
+// their vertex-row gathers hit the same cache lines.  Blocks are remapped so
+// that each XCD (workgroups are dealt round-robin over the 8 XCDs) renders
+// one contiguous slab of tiles and keeps its L2 footprint local.
+//
+// Adjoint: the forward keeps nothing per sample.  The backward walks every
+// ray front to back again, rebuilding e, alpha, T and the prefix
+// sum_{i<=k} w_i rgb_i; the suffix the alpha adjoint needs is
+// saved_colour - prefix, and 1/(1 - alpha_k) = (1 + e_{k+1}) / (1 + e_k)
+// exactly (no division by a vanishing 1 - alpha).
+#include "quadric.h"
+#include "raygen.h"
+#include "shade.h"
+
+namespace qsdvr {
+
+constexpr int kFusedBlock = 256;
+constexpr int kWavesPerBlock = kFusedBlock / kWave;
+
+struct RaySetup {
+    int64_t out;       // output row of the ray, -1 = inactive lane
+    int cam;
+    f3 ori, dir;
+    uint32_t renders;  // render samples (sdf samples = renders + 1), 0 = empty
+    f3 center;         // camera centre (depth reference)
+};
+
+__device__ __forceinline__ int logical_block() {
+    // XCD-aware remap: hardware deals block b to XCD b % 8.
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    if (nb % 8 != 0) return (int)b;
+    return (int)((b % 8) * (nb / 8) + b / 8);
+}
+
+__device__ __forceinline__ RaySetup setup_ray(const qsdvr_camera *__restrict__ cams, int n_cams,
+                                              uint32_t W, uint32_t H,
+                                              const int32_t *__restrict__ ray_pixel,
+                                              int64_t n_rays, float interval) {
+    RaySetup r;
+    r.out = -1;
+    r.renders = 0;
+    const int lane = threadIdx.x % kWave;
+    const int64_t wave = (int64_t)logical_block() * kWavesPerBlock + threadIdx.x / kWave;
+    int cam = 0;
+    uint32_t pix = 0;
+    if (ray_pixel == nullptr) {
+        const int64_t tx = (W + 7) / 8, ty = (H + 7) / 8, tiles = tx * ty;
+        if (wave >= tiles * n_cams) return r;
+        cam = (int)(wave / tiles);
+        int64_t t = wave - (int64_t)cam * tiles;
+        uint32_t px = (uint32_t)(t % tx) * 8 + (lane % 8);
+        uint32_t py = (uint32_t)(t / tx) * 8 + (lane / 8);
+        if (px >= W || py >= H) return r;
+        pix = py * W + px;
+        r.out = (int64_t)cam * W * H + pix;
+    } else {
+        int64_t i = wave * kWave + lane;
+        if (i >= n_rays) return r;
+        int64_t code = (uint32_t)ray_pixel[i];
+        cam = (int)(code / ((int64_t)W * H));
+        if (cam >= n_cams) return r;
+        pix = (uint32_t)(code - (int64_t)cam * W * H);
+        r.out = i;
+    }
+    const qsdvr_camera &c = cams[cam];
+    if (c.width != W || c.height != H) return r;  // inconsistent camera: lane stays idle
+    RayCutResult cut = ray_cut_pixel(c, pix, interval);
+    r.cam = cam;
+    r.ori = cut.origin;
+    r.dir = cut.dir;
+    r.renders = cut.count < 2 ? 0u : cut.count - 1;
+    r.center = f3{c.translation[0], c.translation[1], c.translation[2]};
+    return r;
+}
+
+// Weighted sum of the 8 vertex rows of a cell (render_torch.py:128-143 as
+// sum_k w_k row_k).  Rows: base + dx + dy R + dz R^2.
+template <int C>
+__device__ __forceinline__ void gather_interp(const float *__restrict__ data, int64_t base,
+                                              int R, const float w[8], float d[C]) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) d[c] = 0.0f;
+    const int64_t RR = (int64_t)R * R;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int64_t row = base + (k & 1) + ((k >> 1) & 1) * (int64_t)R + (k >> 2) * RR;
+        const float *src = data + row * C;
+        if constexpr (C % 4 == 0) {
+            const float4 *s4 = reinterpret_cast<const float4 *>(src);
+#pragma unroll
+            for (int q = 0; q < C / 4; ++q) {
+                float4 v = s4[q];
+                d[4 * q + 0] += w[k] * v.x;
+                d[4 * q + 1] += w[k] * v.y;
+                d[4 * q + 2] += w[k] * v.z;
+                d[4 * q + 3] += w[k] * v.w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < C; ++q) d[q] += w[k] * src[q];
+        }
+    }
+}
+
+// Cells produced by the ray walk always lie in [0, R)^3 (every sample is
+// strictly inside the +-0.9999 cube); the clamp only guards degenerate camera
+// input (NaN / inf) against out-of-bounds gathers.
+__device__ __forceinline__ void clamp_cell(CellPoint &cp, int R) {
+    if ((unsigned)cp.ix >= (unsigned)R || (unsigned)cp.iy >= (unsigned)R ||
+        (unsigned)cp.iz >= (unsigned)R) {
+        cp.ix = min(max(cp.ix, 0), R - 1);
+        cp.iy = min(max(cp.iy, 0), R - 1);
+        cp.iz = min(max(cp.iz, 0), R - 1);
+        cp.cell = cp.ix + cp.iy * R + cp.iz * R * R;
+        cp.local = f3{0.0f, 0.0f, 0.0f};
+    }
+}
+
+struct SampleEval {
+    CellPoint rp;     // render sample cell / local point
+    f3 normal;
+    f3 rgb, spec;
+    float t;          // distance from the camera centre
+};
+
+template <int NC>
+__device__ __forceinline__ void eval_render_sample(const RaySetup &ray, uint32_t j, float interval,
+                                                   float half_reso, int R,
+                                                   const float *__restrict__ qgrid,
+                                                   const float *__restrict__ data, float w[8],
+                                                   float d[3 * NC + 5], SampleEval &e) {
+    constexpr int C = 3 * NC + 5;
+    e.rp = render_sample(ray.ori, ray.dir, j, interval, half_reso, R);
+    clamp_cell(e.rp, R);
+    const Quadric q = load_quadric(qgrid, e.rp.cell);
+    e.normal = quadric_normal(q, e.rp.local);
+    trilinear_weights(e.rp.local, w);
+    gather_interp<C>(data, e.rp.cell, R, w, d);
+    ShadeOut so = shade<NC>(e.normal, ray.dir, d);
+    e.rgb = so.rgb;
+    e.spec = so.spec;
+    const float sj = ((float)j + 0.5f) * interval;
+    const float px = ray.ori.x + ray.dir.x * sj - ray.center.x;
+    const float py = ray.ori.y + ray.dir.y * sj - ray.center.y;
+    const float pz = ray.ori.z + ray.dir.z * sj - ray.center.z;
+    e.t = sqrtf(px * px + py * py + pz * pz);
+}
+
+__device__ __forceinline__ float sdf_at(const RaySetup &ray, uint32_t j, float interval,
+                                        float half_reso, int R, float inv_reso,
+                                        const float *__restrict__ qgrid, CellPoint &cp) {
+    cp = sdf_sample(ray.ori, ray.dir, j, interval, half_reso, R);
+    clamp_cell(cp, R);
+    return quadric_sdf(load_quadric(qgrid, cp.cell), cp.local, inv_reso);
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
+    const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
+    const int32_t *__restrict__ ray_pixel,
+    int64_t n_rays, const float *__restrict__ qgrid, const float *__restrict__ data, int R,
+    float interval, float s, float *__restrict__ color, float *__restrict__ depth) {
+    constexpr int C = 3 * NC + 5;
+    RaySetup ray = setup_ray(cams, n_cams, W, H, ray_pixel, n_rays, interval);
+    if (ray.out < 0) return;
+    const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
+    f3 col{0.0f, 0.0f, 0.0f};
+    float dep = 0.0f;
+    if (ray.renders > 0) {
+        CellPoint cp;
+        float e0 = expf(-s * sdf_at(ray, 0, interval, half_reso, R, inv_reso, qgrid, cp));
+        float T = 1.0f;
+        for (uint32_t j = 0; j < ray.renders; ++j) {
+            float e1 = expf(-s * sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp));
+            float a = fmaxf((e1 - e0) / (1.0f + e1), 0.0f);
+            float w[8], d[C];
+            SampleEval ev;
+            eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
+            float wt = a * T;
+            col.x += ev.rgb.x * wt;
+            col.y += ev.rgb.y * wt;
+            col.z += ev.rgb.z * wt;
+            dep += ev.t * wt;
+            T = T * (1.0f - a);
+            e0 = e1;
+        }
+    }
+    store3(color, ray.out, col);
+    if (depth) depth[ray.out] = dep;
+}
+
+__device__ __forceinline__ void scatter_quadric(float *__restrict__ g, int32_t cell,
+                                                const float v[7], int k_count) {
+    float *dst = g + 7 * (int64_t)cell;
+    for (int k = 0; k < k_count; ++k) atomic_addf(dst + k, v[k]);
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kFusedBlock) fused_bwd_kernel(
+    const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
+    const int32_t *__restrict__ ray_pixel,
+    int64_t n_rays, const float *__restrict__ qgrid, const float *__restrict__ data, int R,
+    float interval, float s, const float *__restrict__ color, const float *__restrict__ depth,
+    const float *__restrict__ gcolor, const float *__restrict__ gdepth,
+    float *__restrict__ gq, float *__restrict__ gdata) {
+    constexpr int C = 3 * NC + 5;
+    RaySetup ray = setup_ray(cams, n_cams, W, H, ray_pixel, n_rays, interval);
+    if (ray.out < 0 || ray.renders == 0) return;
+    const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
+    const f3 Cs = load3(color, ray.out);
+    const f3 gC = load3(gcolor, ray.out);
+    const float Ds = depth ? depth[ray.out] : 0.0f;
+    const float gD = gdepth ? gdepth[ray.out] : 0.0f;
+    const int64_t RR = (int64_t)R * R;
+
+    CellPoint cp0, cp1;
+    float sdf0 = sdf_at(ray, 0, interval, half_reso, R, inv_reso, qgrid, cp0);
+    float e0 = expf(-s * sdf0);
+    float T = 1.0f;
+    f3 pre{0.0f, 0.0f, 0.0f};
+    float pre_d = 0.0f;
+    float carry = 0.0f;  // gradient of sdf sample j from alpha_{j-1}
+    for (uint32_t j = 0; j < ray.renders; ++j) {
+        float sdf1 = sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp1);
+        float e1 = expf(-s * sdf1);
+        float inv1 = 1.0f / (1.0f + e1);
+        float praw = (e1 - e0) * inv1;
+        float a = fmaxf(praw, 0.0f);
+        float w[8], d[C];
+        SampleEval ev;
+        eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
+        const float wt = a * T;
+        pre.x += ev.rgb.x * wt;
+        pre.y += ev.rgb.y * wt;
+        pre.z += ev.rgb.z * wt;
+        pre_d += ev.t * wt;
+        // dL/dalpha_j = T_j (gC.rgb_j + gD t_j) - (gC.S_j + gD SD_j) / (1 - alpha_j)
+        float g_sdf0 = carry, g_sdf1 = 0.0f;
+        if (praw >= 0.0f) {
+            float one_over = (1.0f + e1) / (1.0f + e0);
+            float suffix = gC.x * (Cs.x - pre.x) + gC.y * (Cs.y - pre.y) + gC.z * (Cs.z - pre.z) +
+                           gD * (Ds - pre_d);
+            float here = gC.x * ev.rgb.x + gC.y * ev.rgb.y + gC.z * ev.rgb.z + gD * ev.t;
+            float da = T * here - suffix * one_over;
+            g_sdf0 += da * s * e0 * inv1;
+            g_sdf1 = -da * s * e1 * (1.0f + e0) * inv1 * inv1;
+        }
+        // sdf sample j is final: scatter its quadric gradient
+        {
+            float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
+            quadric_sdf_grad(load_quadric(qgrid, cp0.cell), cp0.local, g_sdf0, inv_reso, gq7);
+            scatter_quadric(gq, cp0.cell, gq7, 7);
+        }
+        carry = g_sdf1;
+        // rgb_j gradient -> shader adjoint -> normal and appearance grads
+        f3 grgb{gC.x * wt, gC.y * wt, gC.z * wt};
+        float dd[C];
+        f3 gn = shade_vjp<NC>(ev.normal, ray.dir, d, ev.spec, grgb, dd);
+        {
+            float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
+            quadric_normal_grad(load_quadric(qgrid, ev.rp.cell), ev.rp.local, gn, gq7);
+            scatter_quadric(gq, ev.rp.cell, gq7, 6);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int64_t row = (int64_t)ev.rp.cell + (k & 1) + ((k >> 1) & 1) * (int64_t)R +
+                                (k >> 2) * RR;
+            float *dst = gdata + row * C;
+#pragma unroll
+            for (int c = 0; c < C; ++c) atomic_addf(dst + c, w[k] * dd[c]);
+        }
+        T = T * (1.0f - a);
+        e0 = e1;
+        cp0 = cp1;
+    }
+    {
+        float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
+        quadric_sdf_grad(load_quadric(qgrid, cp0.cell), cp0.local, carry, inv_reso, gq7);
+        scatter_quadric(gq, cp0.cell, gq7, 7);
+    }
+}
+
+static int64_t fused_waves(uint32_t W, uint32_t H, int n_cams, const int32_t *ray_pixel,
+                           int64_t n_rays) {
+    if (ray_pixel == nullptr) {
+        int64_t tx = (W + 7) / 8, ty = (H + 7) / 8;
+        return tx * ty * n_cams;
+    }
+    return (n_rays + kWave - 1) / kWave;
+}
+
+}  // namespace qsdvr
+
+using namespace qsdvr;
+
+static int fused_check(const qsdvr_camera *cams, int32_t n_cams, int32_t width, int32_t height,
+                       const int32_t *ray_pixel, int64_t n_rays, const float *sdf_grid,
+                       const float *data_grid, int32_t reso, int32_t channels, float interval) {
+    QSDVR_REQUIRE(cams && sdf_grid && data_grid, "qsdvr_fused: null pointer argument");
+    QSDVR_REQUIRE(n_cams > 0 && n_rays >= 0, "qsdvr_fused: bad camera / ray counts");
+    QSDVR_REQUIRE(width > 0 && height > 0, "qsdvr_fused: empty image");
+    QSDVR_REQUIRE(reso > 0 && reso <= 1290, "qsdvr_fused: reso %d out of range", reso);
+    QSDVR_REQUIRE(channels == 32 || channels == 53, "qsdvr_fused: channels must be 32 or 53");
+    QSDVR_REQUIRE(interval > 0.0f, "qsdvr_fused: interval must be > 0");
+    QSDVR_REQUIRE((int64_t)n_cams * width * height <= 0x7FFFFFFFll,
+                  "qsdvr_fused: n_cams*W*H exceeds the 31-bit ray code");
+    if (ray_pixel == nullptr)
+        QSDVR_REQUIRE(n_rays == (int64_t)n_cams * width * height,
+                      "qsdvr_fused: n_rays must be n_cams*W*H when ray_pixel is NULL");
+    return QSDVR_OK;
+}
+
+#define QSDVR_FUSED_DISPATCH(KERNEL, ...)                                               \
+    do {                                                                                \
+        int64_t waves = fused_waves(width, height, n_cams, ray_pixel, n_rays);          \
+        unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);    \
+        if (channels == 32)                                                             \
+            KERNEL<9><<<blocks, kFusedBlock, 0, s>>>(__VA_ARGS__);                      \
+        else                                                                            \
+            KERNEL<16><<<blocks, kFusedBlock, 0, s>>>(__VA_ARGS__);                     \
+        QSDVR_LAUNCHED();                                                               \
+    } while (0)
+
+extern "C" {
+
+int qsdvr_fused_forward(const qsdvr_camera *cams, int32_t n_cams, int32_t width, int32_t height,
+                        const int32_t *ray_pixel, int64_t n_rays, const float *sdf_grid,
+                        const float *data_grid, int32_t reso, int32_t channels, float interval,
+                        float logistic_coef, float *color, float *depth, qsdvr_stream stream) {
+    int st = fused_check(cams, n_cams, width, height, ray_pixel, n_rays, sdf_grid, data_grid,
+                         reso, channels, interval);
+    if (st != QSDVR_OK) return st;
+    QSDVR_REQUIRE(color != nullptr, "qsdvr_fused_forward: null color");
+    if (n_rays == 0) return QSDVR_OK;
+    hipStream_t s = as_stream(stream);
+    QSDVR_FUSED_DISPATCH(fused_fwd_kernel, cams, n_cams, (uint32_t)width, (uint32_t)height,
+                         ray_pixel, n_rays, sdf_grid, data_grid, reso, interval, logistic_coef,
+                         color, depth);
+    return QSDVR_OK;
+}
+
+int qsdvr_fused_backward(const qsdvr_camera *cams, int32_t n_cams, int32_t width, int32_t height,
+                         const int32_t *ray_pixel, int64_t n_rays, const float *sdf_grid,
+                         const float *data_grid, int32_t reso, int32_t channels, float interval,
+                         float logistic_coef, const float *color, const float *depth,
+                         const float *grad_color, const float *grad_depth, float *sdf_grid_grad,
+                         float *data_grid_grad, int32_t scatter_mode, qsdvr_stream stream) {
+    int st = fused_check(cams, n_cams, width, height, ray_pixel, n_rays, sdf_grid, data_grid,
+                         reso, channels, interval);
+    if (st != QSDVR_OK) return st;
+    QSDVR_REQUIRE(color && grad_color && sdf_grid_grad && data_grid_grad,
+                  "qsdvr_fused_backward: null pointer argument");
+    QSDVR_REQUIRE((grad_depth == nullptr) || (depth != nullptr),
+                  "qsdvr_fused_backward: grad_depth needs the saved depth");
+    QSDVR_REQUIRE(scatter_mode == 0 || scatter_mode == 1,
+                  "qsdvr_fused_backward: scatter_mode must be 0 or 1");
+    if (n_rays == 0) return QSDVR_OK;
+    hipStream_t s = as_stream(stream);
+    const float *dsave = grad_depth ? depth : nullptr;
+    QSDVR_FUSED_DISPATCH(fused_bwd_kernel, cams, n_cams, (uint32_t)width, (uint32_t)height,
+                         ray_pixel, n_rays, sdf_grid, data_grid, reso, interval, logistic_coef,
+                         color, dsave, grad_color, grad_depth, sdf_grid_grad, data_grid_grad);
+    return QSDVR_OK;
+}
+
+}  // extern "C"

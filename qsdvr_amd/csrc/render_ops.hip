@@ -1,0 +1,310 @@
+// Op-surface appearance kernels for gfx950: GridInterpolation, Shader and
+// RayAggregate, forward and backward.
+// Reference semantics: models/render_torch.py:117-195 (the oracle), with the
+// reference kernels of kernel/render.cu:98-363 as the operator surface.
+#include "shade.h"
+
+namespace qsdvr {
+
+// ------------------------------------------------- GridInterpolation ----
+// One lane per (sample, channel): a wavefront reads whole vertex rows
+// (C consecutive floats) per gather, so every row load is one coalesced
+// burst and every output row is written contiguously.
+template <int CFIX>
+__global__ void __launch_bounds__(256) interp_fwd_kernel(const float *__restrict__ data,
+                                                         const float *__restrict__ pts,
+                                                         const int32_t *__restrict__ idx,
+                                                         float *__restrict__ out, int64_t n,
+                                                         int R, int Cdyn) {
+    const int C = CFIX > 0 ? CFIX : Cdyn;
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n * C) return;
+    int64_t s = id / C;
+    int c = (int)(id - s * C);
+    int64_t cell = idx[s];
+    int64_t cells = (int64_t)R * R * R;
+    if (cell < 0 || cell >= cells) {
+        out[id] = __builtin_nanf("");
+        return;
+    }
+    f3 p = load3(pts, s);
+    const int64_t RR = (int64_t)R * R;
+    const float *b = data + cell * C + c;
+    auto D = [&](int64_t off) { return b[off * C]; };
+    float a00 = lerp_ref(D(0), D(1), p.x);
+    float a01 = lerp_ref(D(R), D(R + 1), p.x);
+    float a0 = lerp_ref(a00, a01, p.y);
+    float a10 = lerp_ref(D(RR), D(RR + 1), p.x);
+    float a11 = lerp_ref(D(RR + R), D(RR + R + 1), p.x);
+    float a1 = lerp_ref(a10, a11, p.y);
+    out[id] = lerp_ref(a0, a1, p.z);
+}
+
+template <int CFIX>
+__global__ void __launch_bounds__(256) interp_bwd_kernel(float *__restrict__ gdata,
+                                                         const float *__restrict__ pts,
+                                                         const int32_t *__restrict__ idx,
+                                                         const float *__restrict__ up, int64_t n,
+                                                         int R, int Cdyn) {
+    const int C = CFIX > 0 ? CFIX : Cdyn;
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n * C) return;
+    int64_t s = id / C;
+    int c = (int)(id - s * C);
+    int64_t cell = idx[s];
+    if (cell < 0 || cell >= (int64_t)R * R * R) return;
+    f3 p = load3(pts, s);
+    float w[8];
+    trilinear_weights(p, w);
+    float g = up[id];
+    const int64_t RR = (int64_t)R * R;
+    float *b = gdata + cell * C + c;
+    const int64_t off[8] = {0, 1, R, R + 1, RR, RR + 1, RR + R, RR + R + 1};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) atomic_addf(b + off[k] * C, w[k] * g);
+}
+
+// ------------------------------------------------------------- Shader ----
+template <int NC>
+__global__ void __launch_bounds__(256) shader_fwd_kernel(float *__restrict__ out,
+                                                         const float *__restrict__ normal,
+                                                         const float *__restrict__ view,
+                                                         const float *__restrict__ data,
+                                                         float *__restrict__ spec, int64_t n) {
+    constexpr int C = 3 * NC + 5;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float d[C];
+    const float *row = data + i * C;
+#pragma unroll
+    for (int k = 0; k < C; ++k) d[k] = row[k];
+    ShadeOut o = shade<NC>(load3(normal, i), load3(view, i), d);
+    store3(out, i, o.rgb);
+    store3(spec, i, o.spec);
+}
+
+template <int NC>
+__global__ void __launch_bounds__(256) shader_bwd_kernel(const float *__restrict__ up,
+                                                         const float *__restrict__ normal,
+                                                         const float *__restrict__ view,
+                                                         const float *__restrict__ data,
+                                                         const float *__restrict__ spec,
+                                                         float *__restrict__ gnormal,
+                                                         float *__restrict__ gdata, int64_t n) {
+    constexpr int C = 3 * NC + 5;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float d[C], dd[C];
+    const float *row = data + i * C;
+#pragma unroll
+    for (int k = 0; k < C; ++k) d[k] = row[k];
+    f3 gn = shade_vjp<NC>(load3(normal, i), load3(view, i), d, load3(spec, i), load3(up, i), dd);
+    store3(gnormal, i, gn);
+    float *orow = gdata + i * C;
+#pragma unroll
+    for (int k = 0; k < C; ++k) orow[k] = dd[k];
+}
+
+// -------------------------------------------------------- RayAggregate ----
+// NeuS logistic compositing (render_torch.py:155-167), one lane per ray.
+__device__ __forceinline__ bool ray_ok(int4 r, int64_t sdf_total, int64_t render_total) {
+    uint32_t sc = (uint32_t)r.x, so = (uint32_t)r.y, rc = (uint32_t)r.z, ro = (uint32_t)r.w;
+    if (rc == 0) return true;
+    return (int64_t)so + rc + 1 <= sdf_total && (int64_t)ro + rc <= render_total && sc >= rc;
+}
+
+__global__ void __launch_bounds__(256) aggregate_fwd_kernel(
+    float *__restrict__ out, const float *__restrict__ rgb, const float *__restrict__ sdf,
+    const int32_t *__restrict__ ray, float *__restrict__ alpha, float *__restrict__ T_out,
+    int64_t rays, int64_t sdf_total, int64_t render_total, float s) {
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= rays) return;
+    int4 r = reinterpret_cast<const int4 *>(ray)[id];
+    f3 col{0.0f, 0.0f, 0.0f};
+    if (!ray_ok(r, sdf_total, render_total)) {
+        float q = __builtin_nanf("");
+        store3(out, id, f3{q, q, q});
+        return;
+    }
+    uint32_t sc = (uint32_t)r.x, so = (uint32_t)r.y, rc = (uint32_t)r.z, ro = (uint32_t)r.w;
+    if (sc != 0 && rc != 0) {
+        float e0 = expf(-s * sdf[so]);
+        float T = 1.0f;
+        for (uint32_t i = 0; i < rc; ++i) {
+            float e1 = expf(-s * sdf[(int64_t)so + i + 1]);
+            float a = fminf(fmaxf((e1 - e0) / (1.0f + e1), 0.0f), 10.0f);
+            alpha[(int64_t)ro + i] = a;
+            T_out[(int64_t)ro + i] = T;
+            f3 c = load3(rgb, (int64_t)ro + i);
+            col.x += c.x * a * T;
+            col.y += c.y * a * T;
+            col.z += c.z * a * T;
+            e0 = e1;
+            T = T * (1.0f - a);
+        }
+    }
+    store3(out, id, col);
+}
+
+// Adjoint with the saved alpha and T.  dL/dalpha_k = T_k (g.rgb_k - rho_k)
+// where rho is the suffix colour seen after sample k, built back to front by
+// rho_{k-1} = alpha_k (g.rgb_k) + (1 - alpha_k) rho_k -- no division by
+// (1 - alpha).  The clamp passes the gradient wherever the pre-clamp value
+// is >= 0 (torch.clamp semantics).
+__global__ void __launch_bounds__(256) aggregate_bwd_kernel(
+    const float *__restrict__ up, const float *__restrict__ rgb, const float *__restrict__ sdf,
+    const int32_t *__restrict__ ray, const float *__restrict__ alpha,
+    const float *__restrict__ T_in, float *__restrict__ grgb, float *__restrict__ gsdf,
+    int64_t rays, int64_t sdf_total, int64_t render_total, float s) {
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= rays) return;
+    int4 r = reinterpret_cast<const int4 *>(ray)[id];
+    uint32_t sc = (uint32_t)r.x, so = (uint32_t)r.y, rc = (uint32_t)r.z, ro = (uint32_t)r.w;
+    if (sc == 0 || rc == 0 || !ray_ok(r, sdf_total, render_total)) return;
+    f3 g = load3(up, id);
+    float rho = 0.0f;
+    float e1 = expf(-s * sdf[(int64_t)so + rc]);
+    float carry = 0.0f;  // contribution to gsdf[so + k + 1] from alpha_{k+1}
+    for (int64_t k = (int64_t)rc - 1; k >= 0; --k) {
+        float e0 = expf(-s * sdf[so + k]);
+        float a = alpha[ro + k], T = T_in[ro + k];
+        f3 c = load3(rgb, ro + k);
+        float gc = g.x * c.x + g.y * c.y + g.z * c.z;
+        float da = T * (gc - rho);
+        float inv = 1.0f / (1.0f + e1);
+        float pre = (e1 - e0) * inv;
+        float pass = (pre >= 0.0f && pre <= 10.0f) ? da : 0.0f;
+        float d_s0 = pass * s * e0 * inv;                        // d alpha / d sdf_k
+        float d_s1 = -pass * s * e1 * (1.0f + e0) * inv * inv;   // d alpha / d sdf_{k+1}
+        gsdf[so + k + 1] = carry + d_s1;
+        carry = d_s0;
+        float w = a * T;
+        store3(grgb, ro + k, f3{g.x * w, g.y * w, g.z * w});
+        rho = a * gc + (1.0f - a) * rho;
+        e1 = e0;
+    }
+    gsdf[so] = carry;
+}
+
+}  // namespace qsdvr
+
+using namespace qsdvr;
+
+extern "C" {
+
+int qsdvr_grid_interpolation_forward(const float *data_grid, const float *point,
+                                     const int32_t *index, float *out, int64_t n,
+                                     int32_t vertex_reso, int32_t channels, qsdvr_stream stream) {
+    QSDVR_REQUIRE(n >= 0 && vertex_reso >= 2 && channels > 0 && channels <= 1024,
+                  "qsdvr_grid_interpolation_forward: bad sizes");
+    if (n == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(data_grid && point && index && out,
+                  "qsdvr_grid_interpolation_forward: null pointer");
+    int R = vertex_reso - 1;
+    int64_t threads = n * channels;
+    hipStream_t s = as_stream(stream);
+    if (channels == 32)
+        interp_fwd_kernel<32><<<grid_1d(threads, 256), 256, 0, s>>>(data_grid, point, index, out,
+                                                                    n, R, 32);
+    else
+        interp_fwd_kernel<0><<<grid_1d(threads, 256), 256, 0, s>>>(data_grid, point, index, out, n,
+                                                                   R, channels);
+    QSDVR_LAUNCHED();
+    return QSDVR_OK;
+}
+
+int qsdvr_grid_interpolation_backward(float *grad_grid, const float *point, const int32_t *index,
+                                      const float *grad_out, int64_t n, int32_t vertex_reso,
+                                      int32_t channels, qsdvr_stream stream) {
+    QSDVR_REQUIRE(n >= 0 && vertex_reso >= 2 && channels > 0 && channels <= 1024,
+                  "qsdvr_grid_interpolation_backward: bad sizes");
+    if (n == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(grad_grid && point && index && grad_out,
+                  "qsdvr_grid_interpolation_backward: null pointer");
+    int R = vertex_reso - 1;
+    int64_t threads = n * channels;
+    hipStream_t s = as_stream(stream);
+    if (channels == 32)
+        interp_bwd_kernel<32><<<grid_1d(threads, 256), 256, 0, s>>>(grad_grid, point, index,
+                                                                    grad_out, n, R, 32);
+    else
+        interp_bwd_kernel<0><<<grid_1d(threads, 256), 256, 0, s>>>(grad_grid, point, index,
+                                                                   grad_out, n, R, channels);
+    QSDVR_LAUNCHED();
+    return QSDVR_OK;
+}
+
+int qsdvr_shader_forward(float *out, const float *normal, const float *view_dir, const float *data,
+                         float *specular, int64_t n, int32_t channels, qsdvr_stream stream) {
+    QSDVR_REQUIRE(n >= 0, "qsdvr_shader_forward: negative n");
+    QSDVR_REQUIRE(channels == 32 || channels == 53,
+                  "qsdvr_shader_forward: channels must be 32 (SH degree <= 2) or 53 (degree 3)");
+    if (n == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(out && normal && view_dir && data && specular,
+                  "qsdvr_shader_forward: null pointer");
+    hipStream_t s = as_stream(stream);
+    if (channels == 32)
+        shader_fwd_kernel<9><<<grid_1d(n, 256), 256, 0, s>>>(out, normal, view_dir, data, specular,
+                                                             n);
+    else
+        shader_fwd_kernel<16><<<grid_1d(n, 256), 256, 0, s>>>(out, normal, view_dir, data,
+                                                              specular, n);
+    QSDVR_LAUNCHED();
+    return QSDVR_OK;
+}
+
+int qsdvr_shader_backward(const float *grad_out, const float *normal, const float *view_dir,
+                          const float *data, const float *specular, float *normal_grad,
+                          float *data_grad, int64_t n, int32_t channels, qsdvr_stream stream) {
+    QSDVR_REQUIRE(n >= 0, "qsdvr_shader_backward: negative n");
+    QSDVR_REQUIRE(channels == 32 || channels == 53,
+                  "qsdvr_shader_backward: channels must be 32 or 53");
+    if (n == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(grad_out && normal && view_dir && data && specular && normal_grad && data_grad,
+                  "qsdvr_shader_backward: null pointer");
+    hipStream_t s = as_stream(stream);
+    if (channels == 32)
+        shader_bwd_kernel<9><<<grid_1d(n, 256), 256, 0, s>>>(grad_out, normal, view_dir, data,
+                                                             specular, normal_grad, data_grad, n);
+    else
+        shader_bwd_kernel<16><<<grid_1d(n, 256), 256, 0, s>>>(grad_out, normal, view_dir, data,
+                                                              specular, normal_grad, data_grad, n);
+    QSDVR_LAUNCHED();
+    return QSDVR_OK;
+}
+
+int qsdvr_ray_aggregate_forward(float *out, const float *rgb, const float *sdf,
+                                const int32_t *ray_list, float *alpha, float *T,
+                                int64_t ray_count, int64_t sdf_total, int64_t render_total,
+                                float logistic_coef, qsdvr_stream stream) {
+    QSDVR_REQUIRE(ray_count >= 0 && sdf_total >= 0 && render_total >= 0,
+                  "qsdvr_ray_aggregate_forward: negative size");
+    if (ray_count == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(out && ray_list, "qsdvr_ray_aggregate_forward: null pointer");
+    QSDVR_REQUIRE(render_total == 0 || (rgb && sdf && alpha && T),
+                  "qsdvr_ray_aggregate_forward: null sample list");
+    aggregate_fwd_kernel<<<grid_1d(ray_count, 256), 256, 0, as_stream(stream)>>>(
+        out, rgb, sdf, ray_list, alpha, T, ray_count, sdf_total, render_total, logistic_coef);
+    QSDVR_LAUNCHED();
+    return QSDVR_OK;
+}
+
+int qsdvr_ray_aggregate_backward(const float *grad_out, const float *rgb, const float *sdf,
+                                 const int32_t *ray_list, const float *alpha, const float *T,
+                                 float *rgb_grad, float *sdf_grad, int64_t ray_count,
+                                 int64_t sdf_total, int64_t render_total, float logistic_coef,
+                                 qsdvr_stream stream) {
+    QSDVR_REQUIRE(ray_count >= 0 && sdf_total >= 0 && render_total >= 0,
+                  "qsdvr_ray_aggregate_backward: negative size");
+    if (ray_count == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(grad_out && ray_list, "qsdvr_ray_aggregate_backward: null pointer");
+    QSDVR_REQUIRE(render_total == 0 || (rgb && sdf && alpha && T && rgb_grad && sdf_grad),
+                  "qsdvr_ray_aggregate_backward: null sample list");
+    aggregate_bwd_kernel<<<grid_1d(ray_count, 256), 256, 0, as_stream(stream)>>>(
+        grad_out, rgb, sdf, ray_list, alpha, T, rgb_grad, sdf_grad, ray_count, sdf_total,
+        render_total, logistic_coef);
+    QSDVR_LAUNCHED();
+    return QSDVR_OK;
+}
+
+}  // extern "C"

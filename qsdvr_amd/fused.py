@@ -1,0 +1,121 @@
+"""Fused plain renderer: one forward kernel and one recompute-adjoint kernel
+behind a ``torch.autograd.Function``.
+
+Computes the same thing as the op-surface composition of models/render.py:
+225-234 preceded by RayCut + GenerateRayPoints (kernel/camera.cu), for every
+pixel of a batch of cameras (or for an explicit list of rays), but keeps no
+per-sample tensors: the forward saves the per-ray colour and depth only and
+the backward rebuilds alpha and T front to back from them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Any, Sequence
+
+import numpy as np
+import torch
+from torch import autograd
+
+from . import _lib
+from .qsdvr import CameraArgs, _ptr, _stream, _t
+
+SCATTER_TILE = 0     # tile-reduced atomics
+SCATTER_ATOMIC = 1   # one atomic per sample and channel
+
+# When set to a list, every fused kernel launch appends
+# (name, start_event, end_event) recorded on the launching stream.
+TIMERS = None
+
+
+class _timed:
+    def __init__(self, name):
+        self.name = name
+
+    def __enter__(self):
+        if TIMERS is not None:
+            self.a = torch.cuda.Event(enable_timing=True)
+            self.b = torch.cuda.Event(enable_timing=True)
+            self.a.record()
+
+    def __exit__(self, *exc):
+        if TIMERS is not None:
+            self.b.record()
+            TIMERS.append((self.name, self.a, self.b))
+
+
+def camera_buffer(cams: Sequence, device="cuda") -> torch.Tensor:
+    """Pack cameras (``scenes.Camera`` / ``CameraArgs``-like objects) into a
+    device byte tensor holding a ``qsdvr_camera[]`` array."""
+    n = len(cams)
+    arr = (_lib.CameraStruct * n)()
+    for i, c in enumerate(cams):
+        a = c if isinstance(c, CameraArgs) else CameraArgs.from_camera(c)
+        arr[i] = a.to_struct()
+    raw = bytes(arr)
+    return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
+
+
+class FusedRender(autograd.Function):
+    """color [n_rays,3], depth [n_rays] = render(sdf_grid, data_grid).
+
+    Gradients flow to the quadric grid [R,R,R,7] and the appearance grid
+    [(R+1)^3, C] (the reference's trainable tensors)."""
+
+    @staticmethod
+    def forward(ctx: Any, sdf_grid, data_grid, cam_buf, n_cams: int, width: int, height: int,
+                ray_pixel, interval: float, s: float, scatter_mode: int = SCATTER_TILE):
+        R = sdf_grid.shape[0]
+        Cc = data_grid.shape[-1]
+        _t("sdf_grid", sdf_grid, torch.float32)
+        _t("data_grid", data_grid, torch.float32)
+        if sdf_grid.numel() != R ** 3 * 7 or data_grid.numel() != (R + 1) ** 3 * Cc:
+            raise ValueError("grid shapes must be [R,R,R,7] and [(R+1)^3, C]")
+        if ray_pixel is None:
+            n_rays = n_cams * width * height
+        else:
+            _t("ray_pixel", ray_pixel, torch.int32, ndim=1)
+            n_rays = ray_pixel.shape[0]
+        dev = sdf_grid.device
+        color = torch.empty((n_rays, 3), dtype=torch.float32, device=dev)
+        depth = torch.empty((n_rays,), dtype=torch.float32, device=dev)
+        with _timed("fused_forward"):
+            _lib.call("qsdvr_fused_forward", _ptr(cam_buf), n_cams, width, height,
+                      _ptr(ray_pixel), n_rays, _ptr(sdf_grid), _ptr(data_grid), R, Cc,
+                      float(interval), float(s), _ptr(color), _ptr(depth), _stream())
+        ctx.save_for_backward(sdf_grid, data_grid, cam_buf, ray_pixel, color, depth)
+        ctx.args = (n_cams, width, height, n_rays, R, Cc, float(interval), float(s),
+                    int(scatter_mode))
+        ctx.set_materialize_grads(False)
+        return color, depth
+
+    @staticmethod
+    def backward(ctx: Any, gcolor, gdepth):
+        sdf_grid, data_grid, cam_buf, ray_pixel, color, depth = ctx.saved_tensors
+        n_cams, width, height, n_rays, R, Cc, interval, s, mode = ctx.args
+        dev = sdf_grid.device
+        gq = torch.zeros_like(sdf_grid)
+        gd = torch.zeros_like(data_grid)
+        if gcolor is None and gdepth is None:
+            return None, None, None, None, None, None, None, None, None, None
+        if gcolor is None:
+            gcolor = torch.zeros((n_rays, 3), dtype=torch.float32, device=dev)
+        gcolor = gcolor.contiguous()
+        gdep = gdepth.contiguous() if gdepth is not None else None
+        with _timed("fused_backward"):
+            _lib.call("qsdvr_fused_backward", _ptr(cam_buf), n_cams, width, height,
+                      _ptr(ray_pixel), n_rays, _ptr(sdf_grid), _ptr(data_grid), R, Cc, interval,
+                      s, _ptr(color), _ptr(depth), _ptr(gcolor), _ptr(gdep), _ptr(gq), _ptr(gd),
+                      mode, _stream())
+        return gq, gd, None, None, None, None, None, None, None, None
+
+
+def render(sdf_grid, data_grid, cams, interval: float, s: float, ray_pixel=None,
+           scatter_mode: int = SCATTER_TILE, cam_buf=None):
+    """Render every pixel of ``cams`` (list of cameras of one size) or the
+    rays ``ray_pixel`` (int32 codes cam*W*H + pixel).  Returns (color, depth)
+    with one row per ray, row-major per camera."""
+    if cam_buf is None:
+        cam_buf = camera_buffer(cams, sdf_grid.device)
+    W, H = int(cams[0].width), int(cams[0].height)
+    return FusedRender.apply(sdf_grid, data_grid, cam_buf, len(cams), W, H, ray_pixel,
+                             interval, s, scatter_mode)

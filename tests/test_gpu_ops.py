@@ -1,0 +1,179 @@
+"""Per-op parity of the HIP op surface against the float64 oracle
+(models/render_torch.py restated) on the small workload: 64^3 sphere grid,
+100x100 image, 128 samples per ray, SH degree 1 in the 32-channel layout.
+
+Tolerances (scale-aware, see helpers.scale_aware_close): forward values
+1e-5, gradients 1e-4."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import raygen
+from oracle import render as orc
+from qsdvr_amd import qsdvr, scenes
+from qsdvr_amd import render as hr
+from tests.helpers import assert_close
+
+pytestmark = pytest.mark.gpu
+
+FWD, BWD = 1e-5, 1e-4
+
+
+@pytest.fixture(scope="module")
+def wl():
+    w = scenes.WORKLOADS["small"]
+    cam = scenes.orbit_cameras(1, w.width, w.height)[0]
+    smp = raygen.ray_samples(cam.as_dict(), np.float32(w.interval), w.R)
+    grid, data, gen = scenes.build_grids(w, seed=0)
+    target = torch.rand((w.width * w.height, 3), generator=gen)
+    dev = {k: torch.as_tensor(v).cuda() for k, v in smp.items() if isinstance(v, np.ndarray)}
+    return {"w": w, "cam": cam, "smp": smp, "grid": grid, "data": data, "target": target,
+            "dev": dev}
+
+
+def _f64(t):
+    return t.detach().double().cpu().requires_grad_(True)
+
+
+def test_sample_sdf(wl):
+    R = wl["w"].R
+    d = wl["dev"]
+    grid = wl["grid"].cuda()
+    n = d["sdfPointList"].shape[0]
+    out = torch.empty(n, device="cuda")
+    qsdvr.SampleSDFForward(out, grid, d["sdfPointList"], d["sdfIndexList"], R)
+    g64 = _f64(wl["grid"])
+    ref = orc.sample_sdf(g64, torch.as_tensor(wl["smp"]["sdfPointList"]).double(),
+                         torch.as_tensor(wl["smp"]["sdfIndexList"]), R)
+    assert_close(out, ref, FWD, "SampleSDF forward")
+    up = torch.randn(n, generator=torch.Generator().manual_seed(1))
+    (ref * up.double()).sum().backward()
+    gg = torch.zeros_like(grid)
+    qsdvr.SampleSDFBackward(up.cuda(), grid, d["sdfPointList"], d["sdfIndexList"], gg, R)
+    assert_close(gg.reshape(-1, 7), g64.grad.reshape(-1, 7), BWD, "SampleSDF backward")
+
+
+def test_sample_normal(wl):
+    d = wl["dev"]
+    grid = wl["grid"].cuda()
+    n = d["renderPointList"].shape[0]
+    out = torch.empty((n, 3), device="cuda")
+    qsdvr.SampleNormalForward(out, grid, d["renderPointList"], d["renderIndexList"])
+    g64 = _f64(wl["grid"])
+    ref = orc.sample_normal(g64, torch.as_tensor(wl["smp"]["renderPointList"]).double(),
+                            torch.as_tensor(wl["smp"]["renderIndexList"]))
+    assert_close(out, ref, FWD, "SampleNormal forward")
+    up = torch.randn((n, 3), generator=torch.Generator().manual_seed(2))
+    (ref * up.double()).sum().backward()
+    gg = torch.zeros_like(grid)
+    qsdvr.SampleNormalBackward(up.cuda(), grid, d["renderPointList"], d["renderIndexList"], gg)
+    assert_close(gg.reshape(-1, 7), g64.grad.reshape(-1, 7), BWD, "SampleNormal backward")
+
+
+def test_grid_interpolation(wl):
+    R = wl["w"].R
+    d = wl["dev"]
+    data = wl["data"].cuda()
+    n = d["renderPointList"].shape[0]
+    out = torch.empty((n, 32), device="cuda")
+    qsdvr.GridInterpolationForward(data, d["renderPointList"], d["renderIndexList"], out, R + 1)
+    d64 = _f64(wl["data"])
+    ref = orc.grid_interpolation(d64, torch.as_tensor(wl["smp"]["renderPointList"]).double(),
+                                 torch.as_tensor(wl["smp"]["renderIndexList"]), R)
+    assert_close(out, ref, FWD, "GridInterpolation forward")
+    up = torch.randn((n, 32), generator=torch.Generator().manual_seed(3))
+    (ref * up.double()).sum().backward()
+    gg = torch.zeros_like(data)
+    qsdvr.GridInterpolationBackward(gg, d["renderPointList"], d["renderIndexList"], up.cuda(),
+                                    R + 1)
+    assert_close(gg, d64.grad, BWD, "GridInterpolation backward")
+
+
+def test_shader(wl):
+    g = torch.Generator().manual_seed(4)
+    n = 50000
+    nrm = torch.nn.functional.normalize(torch.randn((n, 3), generator=g), dim=1)
+    view = torch.nn.functional.normalize(torch.randn((n, 3), generator=g), dim=1)
+    data = torch.randn((n, 32), generator=g) * 0.3
+    data[:, 27:] = torch.rand((n, 5), generator=g)
+    out = torch.empty((n, 3), device="cuda")
+    spec = torch.empty((n, 3), device="cuda")
+    qsdvr.ShaderForward(out, nrm.cuda(), view.cuda(), data.cuda(), spec)
+    n64, d64 = _f64(nrm), _f64(data)
+    rgb, L = orc.shade(n64, view.double(), d64)
+    assert_close(out, rgb, FWD, "Shader rgb")
+    assert_close(spec, L, FWD, "Shader specular")
+    up = torch.randn((n, 3), generator=g)
+    (rgb * up.double()).sum().backward()
+    gN = torch.empty((n, 3), device="cuda")
+    gD = torch.empty((n, 32), device="cuda")
+    qsdvr.ShaderBackward(up.cuda(), nrm.cuda(), view.cuda(), data.cuda(), spec, gN, gD)
+    assert_close(gN, n64.grad, BWD, "Shader normal grad")
+    assert_close(gD, d64.grad, BWD, "Shader data grad")
+
+
+def _aggregate_inputs(wl):
+    smp = wl["smp"]
+    g = torch.Generator().manual_seed(5)
+    rgb = torch.rand((smp["renderCount"], 3), generator=g)
+    # a smooth sdf along each ray crossing zero, plus noise
+    sdf = torch.randn(smp["sdfCount"], generator=g) * 0.05 + 0.3 - torch.rand(
+        smp["sdfCount"], generator=g) * 0.6
+    return rgb, sdf
+
+
+@pytest.mark.parametrize("s", [5.0, 50.0])
+def test_ray_aggregate(wl, s):
+    smp = wl["smp"]
+    rgb, sdf = _aggregate_inputs(wl)
+    ray = wl["dev"]["rayList"]
+    rays = ray.shape[0]
+    out = torch.empty((rays, 3), device="cuda")
+    alpha = torch.empty(rgb.shape[0], device="cuda")
+    T = torch.empty(rgb.shape[0], device="cuda")
+    qsdvr.RayAggregateForward(out, rgb.cuda(), sdf.cuda(), ray, alpha, T, s)
+    r64, s64 = _f64(rgb), _f64(sdf)
+    col, w = orc.composite(r64, s64, smp["rayList"], s, return_weights=True)
+    assert_close(out, col, FWD, "RayAggregate colour")
+    up = torch.randn((rays, 3), generator=torch.Generator().manual_seed(6))
+    (col * up.double()).sum().backward()
+    gR = torch.empty((rgb.shape[0], 3), device="cuda")
+    gS = torch.empty(sdf.shape[0], device="cuda")
+    qsdvr.RayAggregateBackward(up.cuda(), rgb.cuda(), sdf.cuda(), ray, alpha, T, gR, gS, s)
+    assert_close(gR, r64.grad, BWD, "RayAggregate rgb grad")
+    assert_close(gS, s64.grad, BWD, "RayAggregate sdf grad")
+
+
+def test_layer_to_grid():
+    R = 32
+    x, y, z, off = orc.sphere_layers(R, 0.8, dtype=torch.float32)
+    g = torch.Generator().manual_seed(7)
+    x = x + 0.05 * torch.randn(R, generator=g)
+    y = y + 0.05 * torch.randn(R, generator=g)
+    z = z + 0.05 * torch.randn(R, generator=g)
+    out = torch.empty((R, R, R, 7), device="cuda")
+    qsdvr.LayerToGridForward(out, x, y, z, off)      # CPU layers, as the reference
+    xs = [_f64(t) for t in (x, y, z, off)]
+    ref = orc.layers_to_grid(*xs)
+    assert_close(out.reshape(-1, 7), ref.reshape(-1, 7), FWD, "LayerToGrid forward")
+    up = torch.randn((R, R, R, 7), generator=g)
+    (ref * up.double()).sum().backward()
+    gx, gy, gz, go = (torch.empty(R), torch.empty(R), torch.empty(R), torch.empty(4))
+    qsdvr.LayerToGridBackward(up.cuda(), gx, gy, gz, go)
+    for got, t, nm in ((gx, xs[0], "x"), (gy, xs[1], "y"), (gz, xs[2], "z"), (go, xs[3], "off")):
+        assert_close(got, t.grad, BWD, f"LayerToGrid backward {nm}")
+    # device layers give the same grid
+    out2 = torch.empty_like(out)
+    qsdvr.LayerToGridForward(out2, x.cuda(), y.cuda(), z.cuda(), off.cuda())
+    assert torch.equal(out, out2)
+
+
+def test_invalid_index_yields_nan_not_fault(wl):
+    grid = wl["grid"].cuda()
+    pts = torch.zeros((4, 3), device="cuda")
+    idx = torch.tensor([0, -1, 64 ** 3, 5], dtype=torch.int32, device="cuda")
+    out = torch.empty(4, device="cuda")
+    qsdvr.SampleSDFForward(out, grid, pts, idx, 64)
+    o = out.cpu()
+    assert torch.isfinite(o[0]) and torch.isfinite(o[3])
+    assert torch.isnan(o[1]) and torch.isnan(o[2])
