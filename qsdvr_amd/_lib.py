@@ -72,6 +72,8 @@ def load(path: str | os.PathLike | None = None):
     global _lib
     if _lib is not None and path is None:
         return _lib
+    if path is None and os.environ.get("QSDVR_LIB"):
+        path = os.environ["QSDVR_LIB"]  # an alternative build of the same ABI
     p = Path(path) if path is not None else LIB_PATH
     if not p.exists():
         raise RuntimeError(
@@ -84,8 +86,7 @@ def load(path: str | os.PathLike | None = None):
             continue
         fn.argtypes = args
         fn.restype = res
-    if path is None:
-        _lib = lib
+    _lib = lib
     return lib
 
 

@@ -19,6 +19,8 @@
 #include "raygen.h"
 #include "shade.h"
 
+#include <limits.h>
+
 namespace qsdvr {
 
 constexpr int kFusedBlock = 256;
@@ -39,6 +41,7 @@ __device__ __forceinline__ int logical_block() {
     return (int)((b % 8) * (nb / 8) + b / 8);
 }
 
+template <int WPB = kWavesPerBlock>
 __device__ __forceinline__ RaySetup setup_ray(const qsdvr_camera *__restrict__ cams, int n_cams,
                                               uint32_t W, uint32_t H,
                                               const int32_t *__restrict__ ray_pixel,
@@ -47,7 +50,7 @@ __device__ __forceinline__ RaySetup setup_ray(const qsdvr_camera *__restrict__ c
     r.out = -1;
     r.renders = 0;
     const int lane = threadIdx.x % kWave;
-    const int64_t wave = (int64_t)logical_block() * kWavesPerBlock + threadIdx.x / kWave;
+    const int64_t wave = (int64_t)logical_block() * WPB + threadIdx.x / kWave;
     int cam = 0;
     uint32_t pix = 0;
     if (ray_pixel == nullptr) {
@@ -287,6 +290,277 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_kernel(
     }
 }
 
+
+// ------------------------------------------------ tile-reduced adjoint ----
+// One wavefront per block.  At every step j the 64 rays of the tile touch a
+// small box of cells (typically 2x2x2 to 3x3x3).  Their appearance
+// gradients are summed in a per-wave LDS box accumulator and then flushed
+// with one coalesced global atomic per touched (vertex, channel), line by
+// line of the box (each (z, y) line is contiguous in LDS and in HBM).
+//
+// Conflict-free accumulation: every lane parks its gradient row in its own
+// LDS staging row and, at iteration c, adds channel (c + l) mod C of it
+// (l = lane) to its 8 vertices with a plain read-modify-write.  Within one
+// half-wave the 32 lanes address 32 different channels -- never the same
+// LDS word even when they share a vertex -- so the two half-waves simply take
+// turns.  (This is about 4x faster than LDS float atomics on gfx950, and
+// about 9x faster than one global atomic per sample and channel.)  The 13
+// quadric gradients per sample go through LDS atomics on a cell-box
+// accumulator.  Steps whose box exceeds kCapV vertices (about 4% at
+// 800x800) scatter with direct global atomics in the same rotated order.
+constexpr int kCapV = 64;
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+__device__ __forceinline__ void lds_add(float *p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <int C>
+__device__ __forceinline__ int rotate_channel(int c, int lane) {
+    int ch = c + lane;
+    if constexpr ((C & (C - 1)) == 0) {
+        return ch & (C - 1);
+    } else {
+        if (ch >= C) ch -= C;
+        if (ch >= C) ch -= C;
+        return ch;
+    }
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
+    const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
+    const int32_t *__restrict__ ray_pixel,
+    int64_t n_rays, const float *__restrict__ qgrid, const float *__restrict__ data, int R,
+    float interval, float s, const float *__restrict__ color, const float *__restrict__ depth,
+    const float *__restrict__ gcolor, const float *__restrict__ gdepth,
+    float *__restrict__ gq, float *__restrict__ gdata) {
+    constexpr int C = 3 * NC + 5;
+    constexpr int CS = (C + 31) / 32 * 32;  // LDS row stride: a multiple of the 32 banks
+    extern __shared__ float lds[];
+    float *stage = lds;                 // [64][CS]: per-lane gradient rows
+    float *acc = lds + kWave * CS;      // [kCapV][CS]: vertex-box accumulator
+    float *qacc = acc + kCapV * CS;     // [kCapV][8]: cell-box accumulator (A..G, pad)
+    float *qrow = qacc + kCapV * 8 + threadIdx.x * 16;
+    const int lane = threadIdx.x;
+    for (int i = lane; i < kCapV * CS; i += kWave) acc[i] = 0.0f;
+    for (int i = lane; i < kCapV * 8; i += kWave) qacc[i] = 0.0f;
+
+    RaySetup ray = setup_ray<1>(cams, n_cams, W, H, ray_pixel, n_rays, interval);
+    const bool live = ray.out >= 0 && ray.renders > 0;
+    const int steps = wave_max(live ? (int)ray.renders + 1 : 0);
+    if (steps == 0) return;
+    __syncthreads();
+
+    const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
+    const int64_t RR = (int64_t)R * R;
+    f3 Cs{0, 0, 0}, gC{0, 0, 0};
+    float Ds = 0.0f, gD = 0.0f;
+    CellPoint cp0, cp1;
+    cp0.ix = cp0.iy = cp0.iz = 0;
+    cp0.cell = 0;
+    float e0 = 1.0f;
+    if (live) {
+        Cs = load3(color, ray.out);
+        gC = load3(gcolor, ray.out);
+        Ds = depth ? depth[ray.out] : 0.0f;
+        gD = gdepth ? gdepth[ray.out] : 0.0f;
+        e0 = expf(-s * sdf_at(ray, 0, interval, half_reso, R, inv_reso, qgrid, cp0));
+    }
+    float T = 1.0f;
+    f3 pre{0.0f, 0.0f, 0.0f};
+    float pre_d = 0.0f;
+    float carry = 0.0f;
+
+    for (int j = 0; j < steps; ++j) {
+        const bool act_r = live && j < (int)ray.renders;   // render sample j
+        const bool act_s = live && j <= (int)ray.renders;  // sdf sample j (final gradient)
+        float dd[C], w[8];
+        float gs7[7] = {0, 0, 0, 0, 0, 0, 0}, gn7[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < C; ++c) dd[c] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[k] = 0.0f;
+        CellPoint rp = cp0;
+        float g_sdf0 = carry, g_sdf1 = 0.0f;
+        if (act_r) {
+            float sdf1 = sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp1);
+            float e1 = expf(-s * sdf1);
+            float inv1 = 1.0f / (1.0f + e1);
+            float praw = (e1 - e0) * inv1;
+            float a = fmaxf(praw, 0.0f);
+            float d[C];
+            SampleEval ev;
+            eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
+            const float wt = a * T;
+            pre.x += ev.rgb.x * wt;
+            pre.y += ev.rgb.y * wt;
+            pre.z += ev.rgb.z * wt;
+            pre_d += ev.t * wt;
+            if (praw >= 0.0f) {
+                float one_over = (1.0f + e1) / (1.0f + e0);
+                float suffix = gC.x * (Cs.x - pre.x) + gC.y * (Cs.y - pre.y) +
+                               gC.z * (Cs.z - pre.z) + gD * (Ds - pre_d);
+                float here = gC.x * ev.rgb.x + gC.y * ev.rgb.y + gC.z * ev.rgb.z + gD * ev.t;
+                float da = T * here - suffix * one_over;
+                g_sdf0 += da * s * e0 * inv1;
+                g_sdf1 = -da * s * e1 * (1.0f + e0) * inv1 * inv1;
+            }
+            f3 grgb{gC.x * wt, gC.y * wt, gC.z * wt};
+            f3 gn = shade_vjp<NC>(ev.normal, ray.dir, d, ev.spec, grgb, dd);
+            quadric_normal_grad(load_quadric(qgrid, ev.rp.cell), ev.rp.local, gn, gn7);
+            rp = ev.rp;
+            T = T * (1.0f - a);
+            e0 = e1;
+        }
+        const CellPoint sc = cp0;  // sdf sample j
+        if (act_s) quadric_sdf_grad(load_quadric(qgrid, sc.cell), sc.local, g_sdf0, inv_reso, gs7);
+        if (act_r) {
+            carry = g_sdf1;
+            cp0 = cp1;
+        }
+
+        // bounding box of the cells touched this step
+        int lx = INT_MAX, ly = INT_MAX, lz = INT_MAX, hx = INT_MIN, hy = INT_MIN, hz = INT_MIN;
+        if (act_r) {
+            lx = rp.ix; ly = rp.iy; lz = rp.iz;
+            hx = rp.ix; hy = rp.iy; hz = rp.iz;
+        }
+        if (act_s) {
+            lx = min(lx, sc.ix); ly = min(ly, sc.iy); lz = min(lz, sc.iz);
+            hx = max(hx, sc.ix); hy = max(hy, sc.iy); hz = max(hz, sc.iz);
+        }
+        lx = wave_min(lx); ly = wave_min(ly); lz = wave_min(lz);
+        hx = wave_max(hx); hy = wave_max(hy); hz = wave_max(hz);
+        const int ex = hx - lx + 2, ey = hy - ly + 2, ez = hz - lz + 2;
+        const int sxy = ex * ey;
+        const int nv = sxy * ez;
+        if (hx >= lx && nv <= kCapV) {
+            float *row = stage + lane * CS;
+#pragma unroll
+            for (int c = 0; c < C; ++c) row[c] = dd[c];
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                qrow[k] = gs7[k];
+                qrow[8 + k] = gn7[k];
+            }
+            qrow[7] = 0.0f;
+            qrow[15] = 0.0f;
+            __syncthreads();
+            const int vb = (rp.ix - lx) + (rp.iy - ly) * ex + (rp.iz - lz) * sxy;
+            int base[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                base[k] = (vb + (k & 1) + ((k >> 1) & 1) * ex + (k >> 2) * sxy) * CS;
+            for (int half = 0; half < 2; ++half) {
+                const bool mine = act_r && ((lane >> 5) == half);
+                for (int c = 0; c < C; ++c) {
+                    if (mine) {
+                        const int ch = rotate_channel<C>(c, lane);
+                        const float v = row[ch];
+                        float o[8];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) o[k] = acc[base[k] + ch];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) acc[base[k] + ch] = o[k] + w[k] * v;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            if (act_s) {
+                const int cb = ((sc.ix - lx) + (sc.iy - ly) * ex + (sc.iz - lz) * sxy) * 8;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int kk = (k + lane) & 7;
+                    if (kk < 7) lds_add(qacc + cb + kk, qrow[kk]);
+                }
+            }
+            if (act_r) {
+                const int cb = ((rp.ix - lx) + (rp.iy - ly) * ex + (rp.iz - lz) * sxy) * 8;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int kk = (k + lane) & 7;
+                    if (kk < 6) lds_add(qacc + cb + kk, qrow[8 + kk]);
+                }
+            }
+            __syncthreads();
+            // flush line by line: (sz, sy) rows of the box are contiguous both in
+            // the LDS accumulator and in the global grids
+            for (int sz = 0; sz < ez; ++sz) {
+                for (int sy = 0; sy < ey; ++sy) {
+                    const int sbase = sy * ex + sz * sxy;
+                    const int64_t vrow0 = (int64_t)lx + (int64_t)(ly + sy) * R + (int64_t)(lz + sz) * RR;
+                    for (int i = lane; i < ex * C; i += kWave) {
+                        const int sx = i / C, ch = i - sx * C;
+                        float *a = acc + (sbase + sx) * CS + ch;
+                        const float v = *a;
+                        if (v != 0.0f) {
+                            *a = 0.0f;
+                            atomic_addf(gdata + (vrow0 + sx) * C + ch, v);
+                        }
+                    }
+                }
+            }
+            for (int sz = 0; sz < ez - 1; ++sz) {
+                for (int sy = 0; sy < ey - 1; ++sy) {
+                    const int sbase = sy * ex + sz * sxy;
+                    const int64_t c0 = (int64_t)lx + (int64_t)(ly + sy) * R + (int64_t)(lz + sz) * RR;
+                    for (int i = lane; i < (ex - 1) * 7; i += kWave) {
+                        const int sx = i / 7, k = i - sx * 7;
+                        float *a = qacc + (sbase + sx) * 8 + k;
+                        const float v = *a;
+                        if (v != 0.0f) {
+                            *a = 0.0f;
+                            atomic_addf(gq + (c0 + sx) * 7 + k, v);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        } else {
+            float *row = stage + lane * CS;
+#pragma unroll
+            for (int c = 0; c < C; ++c) row[c] = dd[c];
+            __syncthreads();
+            if (act_r) {
+                int64_t grow[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    grow[k] = ((int64_t)rp.cell + (k & 1) + ((k >> 1) & 1) * (int64_t)R + (k >> 2) * RR) * C;
+#pragma unroll 4
+                for (int c = 0; c < C; ++c) {
+                    const int ch = rotate_channel<C>(c, lane);
+                    const float v = row[ch];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) atomic_addf(gdata + grow[k] + ch, w[k] * v);
+                }
+                scatter_quadric(gq, rp.cell, gn7, 6);
+            }
+            if (act_s) scatter_quadric(gq, sc.cell, gs7, 7);
+            __syncthreads();
+        }
+    }
+}
+
+template <int NC>
+constexpr size_t tile_lds_bytes() {
+    constexpr int C = 3 * NC + 5;
+    constexpr int CS = (C + 31) / 32 * 32;
+    return (size_t)(kWave * CS + kCapV * CS + kCapV * 8 + kWave * 16) * sizeof(float);
+}
+
 static int64_t fused_waves(uint32_t W, uint32_t H, int n_cams, const int32_t *ray_pixel,
                            int64_t n_rays) {
     if (ray_pixel == nullptr) {
@@ -364,9 +638,25 @@ int qsdvr_fused_backward(const qsdvr_camera *cams, int32_t n_cams, int32_t width
     if (n_rays == 0) return QSDVR_OK;
     hipStream_t s = as_stream(stream);
     const float *dsave = grad_depth ? depth : nullptr;
-    QSDVR_FUSED_DISPATCH(fused_bwd_kernel, cams, n_cams, (uint32_t)width, (uint32_t)height,
-                         ray_pixel, n_rays, sdf_grid, data_grid, reso, interval, logistic_coef,
-                         color, dsave, grad_color, grad_depth, sdf_grid_grad, data_grid_grad);
+    if (scatter_mode == 1) {
+        QSDVR_FUSED_DISPATCH(fused_bwd_kernel, cams, n_cams, (uint32_t)width, (uint32_t)height,
+                             ray_pixel, n_rays, sdf_grid, data_grid, reso, interval,
+                             logistic_coef, color, dsave, grad_color, grad_depth, sdf_grid_grad,
+                             data_grid_grad);
+        return QSDVR_OK;
+    }
+    unsigned blocks = (unsigned)fused_waves(width, height, n_cams, ray_pixel, n_rays);
+    if (channels == 32)
+        fused_bwd_tile_kernel<9><<<blocks, kWave, tile_lds_bytes<9>(), s>>>(
+            cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
+            data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth,
+            sdf_grid_grad, data_grid_grad);
+    else
+        fused_bwd_tile_kernel<16><<<blocks, kWave, tile_lds_bytes<16>(), s>>>(
+            cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
+            data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth,
+            sdf_grid_grad, data_grid_grad);
+    QSDVR_LAUNCHED();
     return QSDVR_OK;
 }
 

@@ -101,16 +101,18 @@ def _t(name, t, dtype, device="cuda", ndim=None, shape=None):
         raise ValueError(f"{name}: expected a torch.Tensor, got {type(t).__name__}")
     if t.dtype != dtype:
         raise ValueError(f"{name}: expected dtype {dtype}, got {t.dtype}")
-    if device is not None and t.device.type != device:
-        raise ValueError(f"{name}: expected a {device} tensor, got {t.device}")
     if not t.is_contiguous():
         raise ValueError(f"{name}: tensor must be contiguous")
     if ndim is not None and t.dim() != ndim:
         raise ValueError(f"{name}: expected {ndim} dims, got shape {tuple(t.shape)}")
     if shape is not None:
+        if t.dim() < len(shape):
+            raise ValueError(f"{name}: expected shape {shape}, got {tuple(t.shape)}")
         for i, s in enumerate(shape):
             if s is not None and t.shape[i] != s:
                 raise ValueError(f"{name}: expected shape {shape}, got {tuple(t.shape)}")
+    if device is not None and t.device.type != device:
+        raise ValueError(f"{name}: expected a {device} tensor, got {t.device}")
     return t
 
 
@@ -222,9 +224,10 @@ def RayAggregateBackward(outgrad, rgbList, sdfList, rayList, alphaList, TList, r
 
 # ------------------------------------------------------ surface ops ----
 def _grid7(name, g):
-    _t(name, g, torch.float32)
+    _t(name, g, torch.float32, device=None)
     if g.numel() % 7 != 0 or g.numel() == 0:
         raise ValueError(f"{name}: numel {g.numel()} is not a positive multiple of 7")
+    _t(name, g, torch.float32)
     return g.numel() // 7
 
 

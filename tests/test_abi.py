@@ -1,0 +1,67 @@
+"""The C-ABI library loads and exports every entry point declared in
+include/qsdvr_hip.h; argument validation answers without touching a GPU."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from qsdvr_amd import _lib
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    text = (ROOT / "include" / "qsdvr_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(qsdvr_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_op_surface():
+    names = declared_functions()
+    for n in ("qsdvr_ray_cut", "qsdvr_generate_ray_points", "qsdvr_layer_to_grid_forward",
+              "qsdvr_layer_to_grid_backward", "qsdvr_sample_sdf_forward",
+              "qsdvr_sample_sdf_backward", "qsdvr_sample_normal_forward",
+              "qsdvr_sample_normal_backward", "qsdvr_grid_interpolation_forward",
+              "qsdvr_grid_interpolation_backward", "qsdvr_shader_forward",
+              "qsdvr_shader_backward", "qsdvr_ray_aggregate_forward",
+              "qsdvr_ray_aggregate_backward"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the Python binding knows every one of them
+    assert set(declared_functions()) <= set(_lib.SIGNATURES)
+
+
+def test_version_and_validation_without_gpu():
+    lib = _lib.load()
+    assert lib.qsdvr_abi_version() == 1
+    st = lib.qsdvr_sample_sdf_forward(None, None, None, None, -1, 8, 2.0, None)
+    assert st == -1 and b"bad sizes" in lib.qsdvr_last_error()
+    st = lib.qsdvr_shader_forward(None, None, None, None, None, 10, 31, None)
+    assert st == -1 and b"channels" in lib.qsdvr_last_error()
+    st = lib.qsdvr_generate_ray_points(None, None, None, None, None, None, None, None, 5, 0, 0,
+                                       0.1, 0, None)
+    assert st == -1
+    assert lib.qsdvr_ray_cut_workspace_size(1000) >= 1000
+    assert lib.qsdvr_layer_to_grid_workspace_size(128) >= 6 * 128 * 128 * 4
+    # zero-size work is a no-op success
+    assert lib.qsdvr_sample_sdf_forward(None, None, None, None, 0, 8, 2.0, None) == 0
+
+
+def test_fused_rejects_bad_arguments():
+    lib = _lib.load()
+    st = lib.qsdvr_fused_forward(None, 1, 8, 8, None, 64, None, None, 4, 32, 0.1, 5.0, None,
+                                 None, None)
+    assert st == -1 and b"null" in lib.qsdvr_last_error()
+    dummy = C.c_void_p(16)
+    st = lib.qsdvr_fused_forward(dummy, 1, 8, 8, None, 65, dummy, dummy, 4, 32, 0.1, 5.0,
+                                 dummy, None, None)
+    assert st == -1 and b"n_rays" in lib.qsdvr_last_error()
+    st = lib.qsdvr_fused_forward(dummy, 1, 8, 8, None, 64, dummy, dummy, 4, 33, 0.1, 5.0,
+                                 dummy, None, None)
+    assert st == -1 and b"channels" in lib.qsdvr_last_error()

@@ -1,0 +1,247 @@
+"""Pins the render oracle (oracle/render.py, restating models/render_torch.py)
+with checks that do not re-type its formulas: closed forms, special cases,
+an independent SH table, literal-loop vs vectorised compositing, the NeuS
+telescoping identity, the reference's own reverse recurrences for the
+LayerToGrid adjoint, and float64 finite differences for every gradient."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import render as orc
+
+D = torch.float64
+
+
+def test_sphere_layers_closed_form():
+    # init_as_sphere (render_torch.py:22-28): f = sum_axes (2i + 1 + p - R)^2 - (rR)^2
+    R, r = 6, 0.7
+    g = orc.layers_to_grid(*orc.sphere_layers(R, r))
+    z, y, x = torch.meshgrid(torch.arange(R), torch.arange(R), torch.arange(R), indexing="ij")
+    c = 1 - R
+    for k, ax in ((0, x), (1, y), (2, z)):
+        assert torch.all(g[..., k] == 1)
+        assert torch.allclose(g[..., 3 + k], (2 * (2 * ax + c)).to(D))
+    G = (2 * x + c) ** 2 + (2 * y + c) ** 2 + (2 * z + c) ** 2 - (r * R) ** 2
+    assert torch.allclose(g[..., 6], G.to(D))
+
+
+def _reference_layer_adjoint(grad):
+    """The reverse recurrences of kernel/surface.cu:301-334 applied to the
+    per-axis sums of the grid gradient (the reference's own algorithm)."""
+    R = grad.shape[0]
+    gx = [grad[..., 0].sum((0, 1)), grad[..., 3].sum((0, 1)), grad[..., 6].sum((0, 1))]
+    gy = [grad[..., 1].sum((0, 2)), grad[..., 4].sum((0, 2)), grad[..., 6].sum((0, 2))]
+    gz = [grad[..., 2].sum((1, 2)), grad[..., 5].sum((1, 2)), grad[..., 6].sum((1, 2))]
+    out = []
+    for a, d, c in (gx, gy, gz):
+        a, d, c = a.clone(), d.clone(), c.clone()
+        for i in range(R - 1, 0, -1):
+            a[i] += c[i]
+            a[i - 1] += 3 * c[i]
+            d[i - 1] += 2 * c[i]
+            c[i - 1] += c[i]
+        for i in range(R - 1, 0, -1):
+            a[i] += 2 * d[i]
+            a[i - 1] += 2 * d[i]
+            d[i - 1] += d[i]
+        out.append((a, d[0]))
+    off = torch.stack([out[0][1], out[1][1], out[2][1], grad[..., 6].sum()])
+    return out[0][0], out[1][0], out[2][0], off
+
+
+def test_layer_adjoint_matches_reference_recurrences_and_transpose():
+    R = 7
+    g = torch.Generator().manual_seed(0)
+    ins = [torch.randn(R, generator=g, dtype=D).requires_grad_(True) for _ in range(3)]
+    ins.append(torch.randn(4, generator=g, dtype=D).requires_grad_(True))
+    out = orc.layers_to_grid(*ins)
+    up = torch.randn(out.shape, generator=g, dtype=D)
+    (out * up).sum().backward()
+    ref = _reference_layer_adjoint(up)
+    for a, b in zip([t.grad for t in ins], ref):
+        assert torch.allclose(a, b, rtol=1e-12, atol=1e-9)
+    # linear map: <J v, w> == <v, J^T w>
+    v = [torch.randn(R, generator=g, dtype=D) for _ in range(3)] + [torch.randn(4, generator=g, dtype=D)]
+    lhs = (orc.layers_to_grid(*v) * up).sum()
+    rhs = sum((a * b).sum() for a, b in zip(v, ref))
+    assert torch.allclose(lhs, rhs, rtol=1e-12)
+
+
+def test_sdf_plane_and_sphere_normal():
+    R = 8
+    g = torch.Generator().manual_seed(1)
+    n = 50
+    grid = torch.zeros((R, R, R, 7), dtype=D)
+    grid[..., 3:6] = torch.tensor([0.3, -0.4, 1.2], dtype=D)
+    grid[..., 6] = 0.25
+    pts = torch.rand((n, 3), generator=g, dtype=D) * 2 - 1
+    idx = torch.randint(0, R ** 3, (n,), generator=g)
+    sdf = orc.sample_sdf(grid, pts, idx, R)
+    exp = (pts @ torch.tensor([0.3, -0.4, 1.2], dtype=D) + 0.25) / math.sqrt(0.09 + 0.16 + 1.44) / R
+    assert torch.allclose(sdf, exp, rtol=1e-13)
+    nrm = orc.sample_normal(grid, pts, idx)
+    assert torch.allclose(nrm, torch.tensor([0.3, -0.4, 1.2], dtype=D).expand(n, 3) / math.sqrt(1.69))
+    # sphere quadric: the normal is the radial direction
+    sph = orc.layers_to_grid(*orc.sphere_layers(R, 0.5))
+    cell = torch.randint(0, R, (n, 3), generator=g)
+    idx = cell[:, 0] + cell[:, 1] * R + cell[:, 2] * R * R
+    u = 2 * cell + 1 + pts - R                     # position relative to the centre
+    nrm = orc.sample_normal(sph, pts, idx)
+    assert torch.allclose(nrm, u / u.norm(dim=1, keepdim=True), atol=1e-12)
+    # and f/|grad f| = (rho^2 - r^2) / (2 rho) in cell units
+    rho = u.norm(dim=1)
+    exp = (rho ** 2 - (0.5 * R) ** 2) / (2 * rho) / R
+    assert torch.allclose(orc.sample_sdf(sph, pts, idx, R), exp, atol=1e-12)
+
+
+def test_interpolation_weights_and_transpose():
+    R, C = 5, 4
+    g = torch.Generator().manual_seed(2)
+    n = 40
+    pts = torch.rand((n, 3), generator=g, dtype=D) * 2 - 1
+    idx = torch.randint(0, R ** 3, (n,), generator=g)
+    const = torch.full(((R + 1) ** 3, C), 0.7, dtype=D)
+    assert torch.allclose(orc.grid_interpolation(const, pts, idx, R), torch.full((n, C), 0.7, dtype=D))
+    # indicator rows: output = the weight of that row; rows i + dx + dy R + dz R^2,
+    # weights prod(dx ? p : 1 - p) with the raw local point
+    for dx, dy, dz in ((0, 0, 0), (1, 0, 0), (0, 1, 1), (1, 1, 1)):
+        data = torch.zeros(((R + 1) ** 3, 1), dtype=D)
+        i0 = 2 + 1 * R + 2 * R * R
+        data[i0 + dx + dy * R + dz * R * R] = 1
+        q = torch.zeros(1, dtype=torch.long) + i0
+        p = pts[:1]
+        wexp = ((p[0, 0] if dx else 1 - p[0, 0]) * (p[0, 1] if dy else 1 - p[0, 1])
+                * (p[0, 2] if dz else 1 - p[0, 2]))
+        assert torch.allclose(orc.grid_interpolation(data, p, q, R)[0, 0], wexp)
+    data = torch.randn(((R + 1) ** 3, C), generator=g, dtype=D).requires_grad_(True)
+    out = orc.grid_interpolation(data, pts, idx, R)
+    up = torch.randn(out.shape, generator=g, dtype=D)
+    (out * up).sum().backward()
+    v = torch.randn(data.shape, generator=g, dtype=D)
+    assert torch.allclose((orc.grid_interpolation(v, pts, idx, R) * up).sum(), (v * data.grad).sum())
+
+
+def test_sh_matches_independent_table():
+    g = torch.Generator().manual_seed(3)
+    v = torch.nn.functional.normalize(torch.randn((30, 3), generator=g, dtype=D), dim=1)
+    table = orc.sh_real_closed_form(v)
+    K = torch.as_tensor(orc.SH_FACTOR_F32, dtype=D)
+    for k in range(9):
+        coef = torch.zeros((30, 9, 3), dtype=D)
+        coef[:, k, :] = 1.0
+        got = orc.sh_irradiance(v, coef, factors=K.numpy())[:, 0]
+        assert torch.allclose(got, table[:, k], rtol=1e-7, atol=1e-8), k  # float32 constants
+    # the float32 constants are the float32 roundings of the closed forms
+    exact = [0.5 / math.sqrt(math.pi), math.sqrt(3 / (4 * math.pi)), math.sqrt(15 / (4 * math.pi)),
+             math.sqrt(5 / (16 * math.pi)), math.sqrt(15 / (16 * math.pi))]
+    assert [float(np.float32(x)) for x in exact] == [float(orc.SH_FACTOR_F32[i]) for i in (0, 1, 4, 6, 8)]
+
+
+def test_shading_special_cases():
+    g = torch.Generator().manual_seed(4)
+    n = 20
+    nrm = torch.nn.functional.normalize(torch.randn((n, 3), generator=g, dtype=D), dim=1)
+    view = torch.nn.functional.normalize(torch.randn((n, 3), generator=g, dtype=D), dim=1)
+    data = torch.randn((n, 32), generator=g, dtype=D)
+    data[:, 30] = 0.0
+    rgb, _ = orc.shade(nrm, view, data)
+    assert torch.allclose(rgb, data[:, 27:30] * data[:, 31:32])
+    data = torch.zeros((n, 32), dtype=D)
+    data[:, 0:3] = torch.randn((n, 3), generator=g, dtype=D)
+    data[:, 30] = 1.0
+    data[:, 31] = 2.0
+    rgb, L = orc.shade(nrm, view, data)
+    assert torch.allclose(rgb, 2.0 * float(orc.SH_FACTOR_F32[0]) * data[:, 0:3])
+    # reflection: r = v - 2 (v.n) n keeps |r| = 1 and is mirror-symmetric
+    data = torch.zeros((n, 32), dtype=D)
+    data[:, 3 * 1 + 0] = 1.0  # coefficient 1 (x), red
+    data[:, 30] = 1.0
+    data[:, 31] = 1.0
+    rgb, _ = orc.shade(nrm, view, data)
+    r = view - 2 * (view * nrm).sum(1, keepdim=True) * nrm
+    assert torch.allclose(rgb[:, 0], float(orc.SH_FACTOR_F32[1]) * r[:, 0])
+
+
+def _rays(counts):
+    counts = np.asarray(counts, np.int64)
+    ren = np.maximum(counts - 1, 0)
+    ray = np.stack([counts, np.cumsum(counts) - counts, ren, np.cumsum(ren) - ren], 1)
+    return ray.astype(np.int32)
+
+
+def test_composite_literal_vs_vectorised_and_invariants():
+    g = torch.Generator().manual_seed(5)
+    counts = [0, 2, 5, 9, 0, 3, 12]
+    ray = _rays(counts)
+    S, Rn = sum(counts), sum(max(c - 1, 0) for c in counts)
+    sdf = torch.randn(S, generator=g, dtype=D) * 0.3
+    rgb = torch.rand((Rn, 3), generator=g, dtype=D)
+    for s in (1.0, 5.0, 40.0):
+        a = orc.composite(rgb, sdf, ray, s)
+        b = orc.composite_literal(rgb, sdf, ray, s)
+        assert torch.allclose(a, b, rtol=1e-13, atol=1e-15)
+        _, w = orc.composite(rgb, sdf, ray, s, return_weights=True)
+        assert torch.all(w >= 0) and torch.all(w.sum(1) <= 1 + 1e-12)
+    assert torch.all(orc.composite(rgb, sdf, ray, 5.0)[[0, 4]] == 0)
+
+
+def test_composite_neus_telescoping():
+    # monotonically decreasing sdf: no clamp, T_k = Phi(sdf_k) / Phi(sdf_0),
+    # opacity = 1 - Phi(sdf_n) / Phi(sdf_0), Phi(x) = sigmoid(s x)
+    s = 7.0
+    sdf = torch.linspace(0.8, -0.6, 15, dtype=D)
+    ray = _rays([15])
+    rgb = torch.ones((14, 3), dtype=D)
+    col, w = orc.composite(rgb, sdf, ray, s, return_weights=True)
+    phi = torch.sigmoid(s * sdf)
+    assert torch.allclose(col[0, 0], 1 - phi[-1] / phi[0], rtol=1e-13)
+    T = 1 - torch.cat([torch.zeros(1, dtype=D), torch.cumsum(w[0], 0)[:-1]])  # T_k = 1 - sum_{i<k} w_i
+    assert torch.allclose(phi[:-1] / phi[0], T, rtol=1e-12)
+    # single step closed form
+    sdf2 = torch.tensor([0.1, -0.2], dtype=D)
+    c2 = orc.composite(torch.tensor([[0.2, 0.5, 0.9]], dtype=D), sdf2, _rays([2]), s)
+    e0, e1 = math.exp(-s * 0.1), math.exp(s * 0.2)
+    a = (e1 - e0) / (1 + e1)
+    assert torch.allclose(c2[0], a * torch.tensor([0.2, 0.5, 0.9], dtype=D))
+
+
+def test_clamp_gradient_passes_at_zero():
+    # equal consecutive sdf -> pre-clamp alpha exactly 0; torch.clamp passes
+    # the gradient there (the reference CUDA kernel zeroes it)
+    sdf = torch.tensor([0.2, 0.2, -0.1], dtype=D, requires_grad=True)
+    rgb = torch.tensor([[1.0, 1.0, 1.0], [0.5, 0.5, 0.5]], dtype=D)
+    col = orc.composite(rgb, sdf, _rays([3]), 5.0)
+    col.sum().backward()
+    assert sdf.grad[1].abs() > 0 and sdf.grad[0].abs() > 0
+
+
+@pytest.mark.parametrize("fn", ["sdf", "normal", "interp", "shade", "composite"])
+def test_finite_differences(fn):
+    g = torch.Generator().manual_seed(6)
+    R = 3
+    if fn in ("sdf", "normal"):
+        grid = (torch.randn((R, R, R, 7), generator=g, dtype=D) + torch.tensor(
+            [0, 0, 0, 2, 2, 2, 0], dtype=D)).requires_grad_(True)
+        pts = torch.rand((6, 3), generator=g, dtype=D) * 2 - 1
+        idx = torch.randint(0, R ** 3, (6,), generator=g)
+        f = (lambda q: orc.sample_sdf(q, pts, idx, R)) if fn == "sdf" else (
+            lambda q: orc.sample_normal(q, pts, idx))
+        assert torch.autograd.gradcheck(f, (grid,))
+    elif fn == "interp":
+        data = torch.randn(((R + 1) ** 3, 4), generator=g, dtype=D).requires_grad_(True)
+        pts = torch.rand((6, 3), generator=g, dtype=D) * 2 - 1
+        idx = torch.randint(0, R ** 3, (6,), generator=g)
+        assert torch.autograd.gradcheck(lambda d: orc.grid_interpolation(d, pts, idx, R), (data,))
+    elif fn == "shade":
+        nrm = torch.nn.functional.normalize(torch.randn((5, 3), generator=g, dtype=D), dim=1)
+        view = torch.nn.functional.normalize(torch.randn((5, 3), generator=g, dtype=D), dim=1)
+        data = torch.randn((5, 32), generator=g, dtype=D)
+        assert torch.autograd.gradcheck(lambda n, d: orc.shade(n, view, d)[0],
+                                        (nrm.requires_grad_(True), data.requires_grad_(True)))
+    else:
+        ray = _rays([4, 0, 6])
+        sdf = (torch.randn(10, generator=g, dtype=D) * 0.3).requires_grad_(True)
+        rgb = torch.rand((8, 3), generator=g, dtype=D).requires_grad_(True)
+        assert torch.autograd.gradcheck(lambda r, s: orc.composite(r, s, ray, 5.0), (rgb, sdf))
