@@ -40,9 +40,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--op-steps", type=int, default=2, help="steps of the op-surface path (0: skip)")
-    ap.add_argument("--cpu-rays", type=int, default=16384,
+    ap.add_argument("--cpu-rays", type=int, default=196608,
                     help="rays in the CPU oracle sample (0: skip the CPU baseline)")
     ap.add_argument("--scatter", type=int, default=0, help="fused scatter mode (0 tile, 1 atomic)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="run a plain device copy of this many bytes first (a counter "
+                         "calibration kernel of known traffic for rocprofv3 --pmc runs)")
     return ap.parse_args()
 
 
@@ -79,6 +82,14 @@ def main():
     P = w.width * w.height
     targets = torch.rand((w.n_poses, P, 3), generator=torch.Generator().manual_seed(1)).to(dev)
     log(f"setup {time.time() - t0:.1f}s")
+    if args.copy_bytes > 0:
+        src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_bytes} bytes read + written, 3 launches")
+        del src, dst
     G = grid.requires_grad_(True)
     D = data.requires_grad_(True)
     CAM_BYTES = 80
