@@ -251,6 +251,91 @@ int qsdvr_fused_backward(const qsdvr_camera *cams, int32_t n_cams,
                          float *data_grid_grad, int32_t scatter_mode,
                          qsdvr_stream stream);
 
+
+/* Like qsdvr_fused_forward, and also accumulates every ray's compositing
+ * weights T_i alpha_i into 64 equal bins over its sampled segment:
+ * bins [n_rays, 64] (caller-zeroed).  These are the proposal's training
+ * targets.  No reference counterpart. */
+int qsdvr_fused_forward_bins(const qsdvr_camera *cams, int32_t n_cams,
+                             int32_t width, int32_t height,
+                             const int32_t *ray_pixel, int64_t n_rays,
+                             const float *sdf_grid, const float *data_grid,
+                             int32_t reso, int32_t channels, float interval,
+                             float logistic_coef, float *color, float *depth,
+                             float *bins, qsdvr_stream stream);
+
+/* Per-ray state of the RayCut of every ray (same ray selection as the
+ * fused entry points): ray_state [n_rays, 8] float = (origin xyz, in-cube
+ * step count as uint32 bits, dir xyz, 0).  No reference counterpart. */
+int qsdvr_ray_state(const qsdvr_camera *cams, int32_t n_cams, int32_t width,
+                    int32_t height, const int32_t *ray_pixel, int64_t n_rays,
+                    float interval, float *ray_state, qsdvr_stream stream);
+
+/* Fused render with explicit sample positions: for every ray with at least
+ * two in-cube steps, K sdf samples at origin + dir * tlist[ray, k] (tlist
+ * [n_rays, K], increasing) and K-1 render samples at the interval midpoints;
+ * compositing as in RayAggregate.  tlist == NULL renders the uniform plain
+ * samples from the ray state.  No reference counterpart. */
+int qsdvr_fused_forward_listed(const qsdvr_camera *cams, int32_t n_cams,
+                               int32_t width, int32_t height,
+                               const int32_t *ray_pixel, int64_t n_rays,
+                               const float *ray_state, const float *tlist,
+                               int32_t K, const float *sdf_grid,
+                               const float *data_grid, int32_t reso,
+                               int32_t channels, float interval,
+                               float logistic_coef, float *color,
+                               float *depth, qsdvr_stream stream);
+
+int qsdvr_fused_backward_listed(const qsdvr_camera *cams, int32_t n_cams,
+                                int32_t width, int32_t height,
+                                const int32_t *ray_pixel, int64_t n_rays,
+                                const float *ray_state, const float *tlist,
+                                int32_t K, const float *sdf_grid,
+                                const float *data_grid, int32_t reso,
+                                int32_t channels, float interval,
+                                float logistic_coef, const float *color,
+                                const float *depth, const float *grad_color,
+                                const float *grad_depth, float *sdf_grid_grad,
+                                float *data_grid_grad, int32_t scatter_mode,
+                                qsdvr_stream stream);
+
+/* ---------------------------------------------------------- proposal --- */
+/* No reference counterpart.  Per-point MLP PE(39) -> 64 -> 64 -> 64 -> 1
+ * (ReLU), bfloat16 weights on MFMA with float32 accumulation, evaluated at
+ * the 64 bin centres of every ray.  theta: float32 master parameters
+ * [qsdvr_proposal_param_count()] = W1[64,39] b1 W2[64,64] b2 W3[64,64] b3
+ * w4[64] b4. */
+int32_t qsdvr_proposal_param_count(void);
+size_t qsdvr_proposal_packed_bytes(void);
+/* theta -> packed bf16 MFMA fragments + biases (device buffer of
+ * qsdvr_proposal_packed_bytes()). */
+int qsdvr_proposal_pack(const float *theta, void *packed, qsdvr_stream stream);
+/* logits [n_rays, 64]; with x0, x1, x2, h3 [n_rays*64, 64] non-NULL the
+ * activations are stored for qsdvr_proposal_backward. */
+int qsdvr_proposal_forward(const float *ray_state, int64_t n_rays,
+                           float interval, const void *packed, float *logits,
+                           float *x0, float *x1, float *x2, float *h3,
+                           qsdvr_stream stream);
+size_t qsdvr_proposal_workspace_size(int64_t n_rays);
+/* Gradient [param_count] of sum_rays -sum_b target_b log softmax(logits)_b;
+ * loss [n_rays] per-ray values (nullable).  Deterministic. */
+int qsdvr_proposal_backward(const float *logits, const float *target,
+                            int64_t n_rays, const float *theta,
+                            const float *x0, const float *x1, const float *x2,
+                            const float *h3, void *workspace,
+                            size_t workspace_bytes, float *grad, float *loss,
+                            qsdvr_stream stream);
+/* Adam update of n float32 parameters (bias-corrected, step >= 1). */
+int qsdvr_adam_step(float *theta, const float *grad, float *m, float *v,
+                    int64_t n, float lr, float beta1, float beta2, float eps,
+                    int32_t step, qsdvr_stream stream);
+/* Fine sample positions tlist [n_rays, K]: t = 0, t = L and K-2 inverse-CDF
+ * samples of (1-eps) softmax(logits) + eps/64 over the 64 bins of [0, L],
+ * L = (count-1) * interval. */
+int qsdvr_proposal_sample(const float *logits, const float *ray_state,
+                          int64_t n_rays, float interval, int32_t K, float eps,
+                          float *tlist, qsdvr_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

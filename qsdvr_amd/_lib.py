@@ -62,6 +62,21 @@ SIGNATURES = {
                             C.c_int),
     "qsdvr_fused_backward": ([P, I32, I32, I32, P, I64, P, P, I32, I32, F, F, P, P, P, P, P,
                               P, I32, P], C.c_int),
+    "qsdvr_fused_forward_bins": ([P, I32, I32, I32, P, I64, P, P, I32, I32, F, F, P, P, P, P],
+                                 C.c_int),
+    "qsdvr_ray_state": ([P, I32, I32, I32, P, I64, F, P, P], C.c_int),
+    "qsdvr_fused_forward_listed": ([P, I32, I32, I32, P, I64, P, P, I32, P, P, I32, I32, F, F,
+                                    P, P, P], C.c_int),
+    "qsdvr_fused_backward_listed": ([P, I32, I32, I32, P, I64, P, P, I32, P, P, I32, I32, F, F,
+                                     P, P, P, P, P, P, I32, P], C.c_int),
+    "qsdvr_proposal_param_count": ([], C.c_int32),
+    "qsdvr_proposal_packed_bytes": ([], SZ),
+    "qsdvr_proposal_pack": ([P, P, P], C.c_int),
+    "qsdvr_proposal_forward": ([P, I64, F, P, P, P, P, P, P, P], C.c_int),
+    "qsdvr_proposal_workspace_size": ([I64], SZ),
+    "qsdvr_proposal_backward": ([P, P, I64, P, P, P, P, P, P, SZ, P, P, P], C.c_int),
+    "qsdvr_adam_step": ([P, P, P, P, I64, F, F, F, F, I32, P], C.c_int),
+    "qsdvr_proposal_sample": ([P, P, I64, F, I32, F, P, P], C.c_int),
 }
 
 _lib = None

@@ -32,6 +32,7 @@ struct RaySetup {
     f3 ori, dir;
     uint32_t renders;  // render samples (sdf samples = renders + 1), 0 = empty
     f3 center;         // camera centre (depth reference)
+    const float *tl;   // explicit sample positions t[K] along the ray, or nullptr (uniform)
 };
 
 __device__ __forceinline__ int logical_block() {
@@ -45,10 +46,14 @@ template <int WPB = kWavesPerBlock>
 __device__ __forceinline__ RaySetup setup_ray(const qsdvr_camera *__restrict__ cams, int n_cams,
                                               uint32_t W, uint32_t H,
                                               const int32_t *__restrict__ ray_pixel,
-                                              int64_t n_rays, float interval) {
+                                              int64_t n_rays, float interval,
+                                              const float4 *__restrict__ ray_state = nullptr,
+                                              const float *__restrict__ tlist = nullptr,
+                                              int K = 0) {
     RaySetup r;
     r.out = -1;
     r.renders = 0;
+    r.tl = nullptr;
     const int lane = threadIdx.x % kWave;
     const int64_t wave = (int64_t)logical_block() * WPB + threadIdx.x / kWave;
     int cam = 0;
@@ -74,12 +79,28 @@ __device__ __forceinline__ RaySetup setup_ray(const qsdvr_camera *__restrict__ c
     }
     const qsdvr_camera &c = cams[cam];
     if (c.width != W || c.height != H) return r;  // inconsistent camera: lane stays idle
-    RayCutResult cut = ray_cut_pixel(c, pix, interval);
     r.cam = cam;
-    r.ori = cut.origin;
-    r.dir = cut.dir;
-    r.renders = cut.count < 2 ? 0u : cut.count - 1;
     r.center = f3{c.translation[0], c.translation[1], c.translation[2]};
+    uint32_t count;
+    if (ray_state != nullptr) {
+        // precomputed by qsdvr_ray_state: (origin, count bits), (dir, 0)
+        const float4 a = ray_state[2 * r.out], b = ray_state[2 * r.out + 1];
+        r.ori = f3{a.x, a.y, a.z};
+        r.dir = f3{b.x, b.y, b.z};
+        count = __float_as_uint(a.w);
+    } else {
+        RayCutResult cut = ray_cut_pixel(c, pix, interval);
+        r.ori = cut.origin;
+        r.dir = cut.dir;
+        count = cut.count;
+    }
+    if (count < 2) return r;
+    if (tlist != nullptr) {
+        r.tl = tlist + r.out * (int64_t)K;
+        r.renders = (uint32_t)(K - 1);
+    } else {
+        r.renders = count - 1;
+    }
     return r;
 }
 
@@ -133,6 +154,14 @@ struct SampleEval {
     float t;          // distance from the camera centre
 };
 
+// Cell and local point of the cube position ori + dir * t (listed samples).
+__device__ __forceinline__ CellPoint point_at(const RaySetup &ray, float t, float half_reso, int R) {
+    const float qx = ((ray.ori.x + ray.dir.x * t) + 1.0f) * half_reso;
+    const float qy = ((ray.ori.y + ray.dir.y * t) + 1.0f) * half_reso;
+    const float qz = ((ray.ori.z + ray.dir.z * t) + 1.0f) * half_reso;
+    return to_cell(qx, qy, qz, R);
+}
+
 template <int NC>
 __device__ __forceinline__ void eval_render_sample(const RaySetup &ray, uint32_t j, float interval,
                                                    float half_reso, int R,
@@ -140,7 +169,9 @@ __device__ __forceinline__ void eval_render_sample(const RaySetup &ray, uint32_t
                                                    const float *__restrict__ data, float w[8],
                                                    float d[3 * NC + 5], SampleEval &e) {
     constexpr int C = 3 * NC + 5;
-    e.rp = render_sample(ray.ori, ray.dir, j, interval, half_reso, R);
+    const float sj = ray.tl ? 0.5f * (ray.tl[j] + ray.tl[j + 1]) : ((float)j + 0.5f) * interval;
+    e.rp = ray.tl ? point_at(ray, sj, half_reso, R)
+                  : render_sample(ray.ori, ray.dir, j, interval, half_reso, R);
     clamp_cell(e.rp, R);
     const Quadric q = load_quadric(qgrid, e.rp.cell);
     e.normal = quadric_normal(q, e.rp.local);
@@ -149,7 +180,6 @@ __device__ __forceinline__ void eval_render_sample(const RaySetup &ray, uint32_t
     ShadeOut so = shade<NC>(e.normal, ray.dir, d);
     e.rgb = so.rgb;
     e.spec = so.spec;
-    const float sj = ((float)j + 0.5f) * interval;
     const float px = ray.ori.x + ray.dir.x * sj - ray.center.x;
     const float py = ray.ori.y + ray.dir.y * sj - ray.center.y;
     const float pz = ray.ori.z + ray.dir.z * sj - ray.center.z;
@@ -159,7 +189,8 @@ __device__ __forceinline__ void eval_render_sample(const RaySetup &ray, uint32_t
 __device__ __forceinline__ float sdf_at(const RaySetup &ray, uint32_t j, float interval,
                                         float half_reso, int R, float inv_reso,
                                         const float *__restrict__ qgrid, CellPoint &cp) {
-    cp = sdf_sample(ray.ori, ray.dir, j, interval, half_reso, R);
+    cp = ray.tl ? point_at(ray, ray.tl[j], half_reso, R)
+                : sdf_sample(ray.ori, ray.dir, j, interval, half_reso, R);
     clamp_cell(cp, R);
     return quadric_sdf(load_quadric(qgrid, cp.cell), cp.local, inv_reso);
 }
@@ -169,9 +200,11 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
     const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
     const int32_t *__restrict__ ray_pixel,
     int64_t n_rays, const float *__restrict__ qgrid, const float *__restrict__ data, int R,
-    float interval, float s, float *__restrict__ color, float *__restrict__ depth) {
+    float interval, float s, float *__restrict__ color, float *__restrict__ depth,
+    const float4 *__restrict__ ray_state, const float *__restrict__ tlist, int K,
+    float *__restrict__ bins) {
     constexpr int C = 3 * NC + 5;
-    RaySetup ray = setup_ray(cams, n_cams, W, H, ray_pixel, n_rays, interval);
+    RaySetup ray = setup_ray(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state, tlist, K);
     if (ray.out < 0) return;
     const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
     f3 col{0.0f, 0.0f, 0.0f};
@@ -191,6 +224,11 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
             col.y += ev.rgb.y * wt;
             col.z += ev.rgb.z * wt;
             dep += ev.t * wt;
+            if (bins) {
+                // 64 equal bins over the ray's sampled segment (training targets)
+                int b = (int)(((float)j + 0.5f) / (float)ray.renders * 64.0f);
+                bins[ray.out * 64 + min(b, 63)] += wt;
+            }
             T = T * (1.0f - a);
             e0 = e1;
         }
@@ -212,9 +250,10 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_kernel(
     int64_t n_rays, const float *__restrict__ qgrid, const float *__restrict__ data, int R,
     float interval, float s, const float *__restrict__ color, const float *__restrict__ depth,
     const float *__restrict__ gcolor, const float *__restrict__ gdepth,
-    float *__restrict__ gq, float *__restrict__ gdata) {
+    float *__restrict__ gq, float *__restrict__ gdata, const float4 *__restrict__ ray_state,
+    const float *__restrict__ tlist, int K) {
     constexpr int C = 3 * NC + 5;
-    RaySetup ray = setup_ray(cams, n_cams, W, H, ray_pixel, n_rays, interval);
+    RaySetup ray = setup_ray(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state, tlist, K);
     if (ray.out < 0 || ray.renders == 0) return;
     const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
     const f3 Cs = load3(color, ray.out);
@@ -345,7 +384,8 @@ __global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
     int64_t n_rays, const float *__restrict__ qgrid, const float *__restrict__ data, int R,
     float interval, float s, const float *__restrict__ color, const float *__restrict__ depth,
     const float *__restrict__ gcolor, const float *__restrict__ gdepth,
-    float *__restrict__ gq, float *__restrict__ gdata) {
+    float *__restrict__ gq, float *__restrict__ gdata, const float4 *__restrict__ ray_state,
+    const float *__restrict__ tlist, int K) {
     constexpr int C = 3 * NC + 5;
     constexpr int CS = (C + 31) / 32 * 32;  // LDS row stride: a multiple of the 32 banks
     extern __shared__ float lds[];
@@ -357,7 +397,7 @@ __global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
     for (int i = lane; i < kCapV * CS; i += kWave) acc[i] = 0.0f;
     for (int i = lane; i < kCapV * 8; i += kWave) qacc[i] = 0.0f;
 
-    RaySetup ray = setup_ray<1>(cams, n_cams, W, H, ray_pixel, n_rays, interval);
+    RaySetup ray = setup_ray<1>(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state, tlist, K);
     const bool live = ray.out >= 0 && ray.renders > 0;
     const int steps = wave_max(live ? (int)ray.renders + 1 : 0);
     if (steps == 0) return;
@@ -591,41 +631,65 @@ static int fused_check(const qsdvr_camera *cams, int32_t n_cams, int32_t width, 
     return QSDVR_OK;
 }
 
-#define QSDVR_FUSED_DISPATCH(KERNEL, ...)                                               \
-    do {                                                                                \
-        int64_t waves = fused_waves(width, height, n_cams, ray_pixel, n_rays);          \
-        unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);    \
-        if (channels == 32)                                                             \
-            KERNEL<9><<<blocks, kFusedBlock, 0, s>>>(__VA_ARGS__);                      \
-        else                                                                            \
-            KERNEL<16><<<blocks, kFusedBlock, 0, s>>>(__VA_ARGS__);                     \
-        QSDVR_LAUNCHED();                                                               \
-    } while (0)
+// Per-ray state for the listed-sample and proposal paths: (origin, count
+// as bits), (dir, 0) -- the RayCut result of every ray, bit for bit.
+__global__ void __launch_bounds__(256) ray_state_kernel(const qsdvr_camera *__restrict__ cams,
+                                                        int n_cams, uint32_t W, uint32_t H,
+                                                        const int32_t *__restrict__ ray_pixel,
+                                                        int64_t n_rays, float interval,
+                                                        float4 *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_rays) return;
+    int64_t code = ray_pixel ? (int64_t)(uint32_t)ray_pixel[i] : i;
+    const int cam = (int)(code / ((int64_t)W * H));
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+    float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (cam < n_cams && cams[cam].width == W && cams[cam].height == H) {
+        const uint32_t pix = (uint32_t)(code - (int64_t)cam * W * H);
+        RayCutResult cut = ray_cut_pixel(cams[cam], pix, interval);
+        a = make_float4(cut.origin.x, cut.origin.y, cut.origin.z, __uint_as_float(cut.count));
+        b = make_float4(cut.dir.x, cut.dir.y, cut.dir.z, 0.0f);
+    }
+    out[2 * i] = a;
+    out[2 * i + 1] = b;
+}
 
-extern "C" {
-
-int qsdvr_fused_forward(const qsdvr_camera *cams, int32_t n_cams, int32_t width, int32_t height,
-                        const int32_t *ray_pixel, int64_t n_rays, const float *sdf_grid,
-                        const float *data_grid, int32_t reso, int32_t channels, float interval,
-                        float logistic_coef, float *color, float *depth, qsdvr_stream stream) {
+static int fused_forward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t width,
+                              int32_t height, const int32_t *ray_pixel, int64_t n_rays,
+                              const float *ray_state, const float *tlist, int32_t K,
+                              const float *sdf_grid, const float *data_grid, int32_t reso,
+                              int32_t channels, float interval, float logistic_coef,
+                              float *color, float *depth, float *bins, hipStream_t s) {
     int st = fused_check(cams, n_cams, width, height, ray_pixel, n_rays, sdf_grid, data_grid,
                          reso, channels, interval);
     if (st != QSDVR_OK) return st;
     QSDVR_REQUIRE(color != nullptr, "qsdvr_fused_forward: null color");
+    QSDVR_REQUIRE(tlist == nullptr || (ray_state != nullptr && K >= 2 && K <= 4096),
+                  "qsdvr_fused_forward: listed samples need the ray state and 2 <= K <= 4096");
     if (n_rays == 0) return QSDVR_OK;
-    hipStream_t s = as_stream(stream);
-    QSDVR_FUSED_DISPATCH(fused_fwd_kernel, cams, n_cams, (uint32_t)width, (uint32_t)height,
-                         ray_pixel, n_rays, sdf_grid, data_grid, reso, interval, logistic_coef,
-                         color, depth);
+    const float4 *rs = reinterpret_cast<const float4 *>(ray_state);
+    int64_t waves = fused_waves(width, height, n_cams, ray_pixel, n_rays);
+    unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (channels == 32)
+        fused_fwd_kernel<9><<<blocks, kFusedBlock, 0, s>>>(
+            cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
+            data_grid, reso, interval, logistic_coef, color, depth, rs, tlist, K, bins);
+    else
+        fused_fwd_kernel<16><<<blocks, kFusedBlock, 0, s>>>(
+            cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
+            data_grid, reso, interval, logistic_coef, color, depth, rs, tlist, K, bins);
+    QSDVR_LAUNCHED();
     return QSDVR_OK;
 }
 
-int qsdvr_fused_backward(const qsdvr_camera *cams, int32_t n_cams, int32_t width, int32_t height,
-                         const int32_t *ray_pixel, int64_t n_rays, const float *sdf_grid,
-                         const float *data_grid, int32_t reso, int32_t channels, float interval,
-                         float logistic_coef, const float *color, const float *depth,
-                         const float *grad_color, const float *grad_depth, float *sdf_grid_grad,
-                         float *data_grid_grad, int32_t scatter_mode, qsdvr_stream stream) {
+static int fused_backward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t width,
+                               int32_t height, const int32_t *ray_pixel, int64_t n_rays,
+                               const float *ray_state, const float *tlist, int32_t K,
+                               const float *sdf_grid, const float *data_grid, int32_t reso,
+                               int32_t channels, float interval, float logistic_coef,
+                               const float *color, const float *depth, const float *grad_color,
+                               const float *grad_depth, float *sdf_grid_grad,
+                               float *data_grid_grad, int32_t scatter_mode, hipStream_t s) {
     int st = fused_check(cams, n_cams, width, height, ray_pixel, n_rays, sdf_grid, data_grid,
                          reso, channels, interval);
     if (st != QSDVR_OK) return st;
@@ -635,29 +699,118 @@ int qsdvr_fused_backward(const qsdvr_camera *cams, int32_t n_cams, int32_t width
                   "qsdvr_fused_backward: grad_depth needs the saved depth");
     QSDVR_REQUIRE(scatter_mode == 0 || scatter_mode == 1,
                   "qsdvr_fused_backward: scatter_mode must be 0 or 1");
+    QSDVR_REQUIRE(tlist == nullptr || (ray_state != nullptr && K >= 2 && K <= 4096),
+                  "qsdvr_fused_backward: listed samples need the ray state and 2 <= K <= 4096");
     if (n_rays == 0) return QSDVR_OK;
-    hipStream_t s = as_stream(stream);
+    const float4 *rs = reinterpret_cast<const float4 *>(ray_state);
     const float *dsave = grad_depth ? depth : nullptr;
+    int64_t waves = fused_waves(width, height, n_cams, ray_pixel, n_rays);
     if (scatter_mode == 1) {
-        QSDVR_FUSED_DISPATCH(fused_bwd_kernel, cams, n_cams, (uint32_t)width, (uint32_t)height,
-                             ray_pixel, n_rays, sdf_grid, data_grid, reso, interval,
-                             logistic_coef, color, dsave, grad_color, grad_depth, sdf_grid_grad,
-                             data_grid_grad);
+        unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+        if (channels == 32)
+            fused_bwd_kernel<9><<<blocks, kFusedBlock, 0, s>>>(
+                cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
+                data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth,
+                sdf_grid_grad, data_grid_grad, rs, tlist, K);
+        else
+            fused_bwd_kernel<16><<<blocks, kFusedBlock, 0, s>>>(
+                cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
+                data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth,
+                sdf_grid_grad, data_grid_grad, rs, tlist, K);
+        QSDVR_LAUNCHED();
         return QSDVR_OK;
     }
-    unsigned blocks = (unsigned)fused_waves(width, height, n_cams, ray_pixel, n_rays);
+    unsigned blocks = (unsigned)waves;
     if (channels == 32)
         fused_bwd_tile_kernel<9><<<blocks, kWave, tile_lds_bytes<9>(), s>>>(
             cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
             data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth,
-            sdf_grid_grad, data_grid_grad);
+            sdf_grid_grad, data_grid_grad, rs, tlist, K);
     else
         fused_bwd_tile_kernel<16><<<blocks, kWave, tile_lds_bytes<16>(), s>>>(
             cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
             data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth,
-            sdf_grid_grad, data_grid_grad);
+            sdf_grid_grad, data_grid_grad, rs, tlist, K);
     QSDVR_LAUNCHED();
     return QSDVR_OK;
+}
+
+extern "C" {
+
+int qsdvr_fused_forward(const qsdvr_camera *cams, int32_t n_cams, int32_t width, int32_t height,
+                        const int32_t *ray_pixel, int64_t n_rays, const float *sdf_grid,
+                        const float *data_grid, int32_t reso, int32_t channels, float interval,
+                        float logistic_coef, float *color, float *depth, qsdvr_stream stream) {
+    return fused_forward_impl(cams, n_cams, width, height, ray_pixel, n_rays, nullptr, nullptr, 0,
+                              sdf_grid, data_grid, reso, channels, interval, logistic_coef, color,
+                              depth, nullptr, as_stream(stream));
+}
+
+int qsdvr_fused_forward_bins(const qsdvr_camera *cams, int32_t n_cams, int32_t width,
+                             int32_t height, const int32_t *ray_pixel, int64_t n_rays,
+                             const float *sdf_grid, const float *data_grid, int32_t reso,
+                             int32_t channels, float interval, float logistic_coef, float *color,
+                             float *depth, float *bins, qsdvr_stream stream) {
+    QSDVR_REQUIRE(bins != nullptr, "qsdvr_fused_forward_bins: null bins");
+    return fused_forward_impl(cams, n_cams, width, height, ray_pixel, n_rays, nullptr, nullptr, 0,
+                              sdf_grid, data_grid, reso, channels, interval, logistic_coef, color,
+                              depth, bins, as_stream(stream));
+}
+
+int qsdvr_fused_backward(const qsdvr_camera *cams, int32_t n_cams, int32_t width, int32_t height,
+                         const int32_t *ray_pixel, int64_t n_rays, const float *sdf_grid,
+                         const float *data_grid, int32_t reso, int32_t channels, float interval,
+                         float logistic_coef, const float *color, const float *depth,
+                         const float *grad_color, const float *grad_depth, float *sdf_grid_grad,
+                         float *data_grid_grad, int32_t scatter_mode, qsdvr_stream stream) {
+    return fused_backward_impl(cams, n_cams, width, height, ray_pixel, n_rays, nullptr, nullptr, 0,
+                               sdf_grid, data_grid, reso, channels, interval, logistic_coef,
+                               color, depth, grad_color, grad_depth, sdf_grid_grad,
+                               data_grid_grad, scatter_mode, as_stream(stream));
+}
+
+int qsdvr_ray_state(const qsdvr_camera *cams, int32_t n_cams, int32_t width, int32_t height,
+                    const int32_t *ray_pixel, int64_t n_rays, float interval, float *ray_state,
+                    qsdvr_stream stream) {
+    QSDVR_REQUIRE(cams && ray_state, "qsdvr_ray_state: null pointer argument");
+    QSDVR_REQUIRE(n_cams > 0 && width > 0 && height > 0 && n_rays >= 0 && interval > 0.0f,
+                  "qsdvr_ray_state: bad sizes");
+    QSDVR_REQUIRE(ray_pixel != nullptr || n_rays == (int64_t)n_cams * width * height,
+                  "qsdvr_ray_state: n_rays must be n_cams*W*H when ray_pixel is NULL");
+    if (n_rays == 0) return QSDVR_OK;
+    ray_state_kernel<<<grid_1d(n_rays, 256), 256, 0, as_stream(stream)>>>(
+        cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, interval,
+        reinterpret_cast<float4 *>(ray_state));
+    QSDVR_LAUNCHED();
+    return QSDVR_OK;
+}
+
+int qsdvr_fused_forward_listed(const qsdvr_camera *cams, int32_t n_cams, int32_t width,
+                               int32_t height, const int32_t *ray_pixel, int64_t n_rays,
+                               const float *ray_state, const float *tlist, int32_t K,
+                               const float *sdf_grid, const float *data_grid, int32_t reso,
+                               int32_t channels, float interval, float logistic_coef,
+                               float *color, float *depth, qsdvr_stream stream) {
+    QSDVR_REQUIRE(ray_state != nullptr, "qsdvr_fused_forward_listed: null ray state");
+    return fused_forward_impl(cams, n_cams, width, height, ray_pixel, n_rays, ray_state, tlist, K,
+                              sdf_grid, data_grid, reso, channels, interval, logistic_coef, color,
+                              depth, nullptr, as_stream(stream));
+}
+
+int qsdvr_fused_backward_listed(const qsdvr_camera *cams, int32_t n_cams, int32_t width,
+                                int32_t height, const int32_t *ray_pixel, int64_t n_rays,
+                                const float *ray_state, const float *tlist, int32_t K,
+                                const float *sdf_grid, const float *data_grid, int32_t reso,
+                                int32_t channels, float interval, float logistic_coef,
+                                const float *color, const float *depth, const float *grad_color,
+                                const float *grad_depth, float *sdf_grid_grad,
+                                float *data_grid_grad, int32_t scatter_mode,
+                                qsdvr_stream stream) {
+    QSDVR_REQUIRE(ray_state != nullptr, "qsdvr_fused_backward_listed: null ray state");
+    return fused_backward_impl(cams, n_cams, width, height, ray_pixel, n_rays, ray_state, tlist,
+                               K, sdf_grid, data_grid, reso, channels, interval, logistic_coef,
+                               color, depth, grad_color, grad_depth, sdf_grid_grad,
+                               data_grid_grad, scatter_mode, as_stream(stream));
 }
 
 }  // extern "C"

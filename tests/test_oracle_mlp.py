@@ -1,0 +1,84 @@
+"""Pins the proposal oracle (oracle/mlp.py): positional encoding closed
+form, bf16 rounding, the loss-gradient closed form, float64 finite
+differences of the network, bin targets and sampler invariants."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp as om
+
+
+def test_positional_encoding_closed_form():
+    p = torch.tensor([[0.25, -0.5, 0.125]], dtype=torch.float64)
+    pe = om.positional_encoding(p)[0]
+    assert pe.shape[0] == om.PE_DIM == 39
+    assert torch.allclose(pe[:3], p[0])
+    for k in range(6):
+        a = (2 ** k) * math.pi * p[0]
+        assert torch.allclose(pe[3 + 6 * k:6 + 6 * k], torch.sin(a))
+        assert torch.allclose(pe[6 + 6 * k:9 + 6 * k], torch.cos(a))
+    # sin(pi/4)*... spot values
+    assert abs(float(pe[3]) - math.sin(math.pi / 4)) < 1e-15
+
+
+def test_bf16_rounding_is_round_to_nearest_even():
+    x = torch.tensor([1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -3.0], dtype=torch.float64)
+    r = om.bf16_round(x)
+    assert r.tolist() == [1.0, 1.0 + 2 ** -6, 1.0, -3.0]
+
+
+def test_loss_gradient_closed_form():
+    g = torch.Generator().manual_seed(0)
+    z = torch.randn((5, 64), generator=g, dtype=torch.float64).requires_grad_(True)
+    q = torch.rand((5, 64), generator=g, dtype=torch.float64)
+    om.ray_loss(z, q).backward()
+    assert torch.allclose(z.grad, om.ray_loss_grad(z.detach(), q))
+
+
+def test_network_finite_differences():
+    theta = om.init_params(1).double().requires_grad_(True)
+    p = (torch.rand((6, 3), generator=torch.Generator().manual_seed(2), dtype=torch.float64) * 2 - 1)
+    # without bf16 rounding the network is smooth almost everywhere: check
+    # autograd against central differences on a few random directions
+    f = lambda th: om.forward(th, p, emulate_bf16=False).sum()  # noqa: E731
+    f(theta).backward()
+    g = torch.Generator().manual_seed(3)
+    for _ in range(4):
+        v = torch.randn(theta.shape, generator=g, dtype=torch.float64)
+        h = 1e-6
+        fd = (f(theta.detach() + h * v) - f(theta.detach() - h * v)) / (2 * h)
+        assert abs(float(fd) - float((theta.grad * v).sum())) < 1e-6 * (1 + abs(float(fd)))
+
+
+def test_bin_targets_partition_the_weights():
+    w = torch.rand((3, 10), dtype=torch.float64)
+    rc = torch.tensor([10, 7, 0])
+    q = om.bin_targets(w, rc)
+    assert torch.allclose(q.sum(1), torch.stack([w[0].sum(), w[1, :7].sum(), torch.zeros((), dtype=torch.float64)]))
+    # render sample j of 10 sits at (j+0.5)/10 of the segment: sample 4 in bin 28
+    assert float(q[0, 28]) == float(w[0, 4])
+
+
+def test_sampler_invariants():
+    z = torch.full((1, 64), 0.0, dtype=torch.float64)
+    t = om.fine_samples(z, torch.tensor([2.0]), K=34, eps=0.05)[0]
+    assert t[0] == 0 and t[-1] == 2.0
+    # a uniform pdf gives uniformly spaced interior samples
+    assert torch.allclose(t[1:-1], (torch.arange(32, dtype=torch.float64) + 0.5) / 32 * 2.0)
+    # all mass in bin 10 (up to eps): the interior samples concentrate there
+    z = torch.full((1, 64), -50.0, dtype=torch.float64)
+    z[0, 10] = 50.0
+    t = om.fine_samples(z, torch.tensor([64.0]), K=32, eps=0.05)[0]
+    inside = ((t >= 10) & (t < 11)).sum()
+    assert int(inside) >= 27 and torch.all(t[1:] > t[:-1])
+
+
+def test_composite_t_matches_list_compositing():
+    from oracle import render as orc
+    g = torch.Generator().manual_seed(4)
+    sdf = torch.randn(9, generator=g, dtype=torch.float64) * 0.2
+    rgb = torch.rand((8, 3), generator=g, dtype=torch.float64)
+    ray = np.array([[9, 0, 8, 0]], dtype=np.int32)
+    assert torch.allclose(om.composite_t(rgb, sdf, 6.0), orc.composite(rgb, sdf, ray, 6.0)[0])
