@@ -317,8 +317,11 @@ int qsdvr_proposal_forward(const float *ray_state, int64_t n_rays,
                            float *x0, float *x1, float *x2, float *h3,
                            qsdvr_stream stream);
 size_t qsdvr_proposal_workspace_size(int64_t n_rays);
-/* Gradient [param_count] of sum_rays -sum_b target_b log softmax(logits)_b;
- * loss [n_rays] per-ray values (nullable).  Deterministic. */
+/* Gradient [param_count] of the per-ray log-likelihood of the bin targets
+ * q under the composited proposal weights: with d = softplus(logits),
+ * w_b = exp(-sum_{c<b} d_c) (1 - exp(-d_b)), T_end = exp(-sum d),
+ * loss = -sum_b q_b log w_b - (1 - sum_b q_b) log T_end.  loss [n_rays]
+ * per-ray values (nullable).  Deterministic. */
 int qsdvr_proposal_backward(const float *logits, const float *target,
                             int64_t n_rays, const float *theta,
                             const float *x0, const float *x1, const float *x2,
@@ -330,8 +333,8 @@ int qsdvr_adam_step(float *theta, const float *grad, float *m, float *v,
                     int64_t n, float lr, float beta1, float beta2, float eps,
                     int32_t step, qsdvr_stream stream);
 /* Fine sample positions tlist [n_rays, K]: t = 0, t = L and K-2 inverse-CDF
- * samples of (1-eps) softmax(logits) + eps/64 over the 64 bins of [0, L],
- * L = (count-1) * interval. */
+ * samples of (1-eps) w/sum(w) + eps/64 (w the composited proposal weights
+ * above) over the 64 bins of [0, L], L = (count-1) * interval. */
 int qsdvr_proposal_sample(const float *logits, const float *ray_state,
                           int64_t n_rays, float interval, int32_t K, float eps,
                           float *tlist, qsdvr_stream stream);

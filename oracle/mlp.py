@@ -22,16 +22,25 @@ Bins: a ray whose plain cut has c >= 2 in-cube steps spans
 t in [0, L], L = (c - 1) * interval, from its first in-cube step; bin b is
 [b L/64, (b+1) L/64) and the network is evaluated at the bin centres.
 
-Loss per ray: -sum_b q_b log softmax(z)_b with q_b the per-bin sums of the
-plain renderer's compositing weights (Q = sum_b q_b; the ray's term is
-weighted by Q, i.e. the target distribution is q/Q).
+Proposal weights: the logit is read as a density d_b = softplus(z_b) of
+bin b, composited front to back like the renderer:
+a_b = 1 - exp(-d_b), T_b = exp(-sum_{c<b} d_c), w_b = T_b a_b and the
+pass-through mass T_end = exp(-sum_c d_c).  Occluded bins (behind an
+opaque one) therefore get no mass, which a per-ray softmax cannot express.
 
-Sampler: pdf_b = (1 - eps) softmax(z)_b + eps / 64; fine sdf samples are
+Loss per ray: the categorical log-likelihood of the plain renderer's
+weights over the 64 bins plus the pass-through outcome,
+  -sum_b q_b log w_b - (1 - sum_b q_b) log T_end,
+with q_b the per-bin sums of the plain compositing weights.  Closed form:
+  dL/dz_b = sigmoid(z_b) [ (1 - sum_{c<=b} q_c) - q_b / expm1(d_b) ].
+
+Sampler: pdf_b = (1 - eps) w_b / sum w + eps / 64; fine sdf samples are
 t = 0, t = L and the inverse CDF of the piecewise-constant pdf at
 u_k = (k + 0.5) / (K - 2) for k < K - 2, merged in increasing order.
 
 Pinned by (tests/test_oracle_mlp.py): PE closed form, float64 finite
-differences of the network and loss, the softmax-CE gradient closed form,
+differences of the network, the loss gradient's closed form and its zero at
+the target,
 and sampler invariants (monotone, endpoints, bins with all the mass get
 the interior samples, a uniform pdf gives uniform spacing).
 """
@@ -119,14 +128,27 @@ def bin_centres(ori, dirs, count, interval, nbins=NBINS):
     return ori[:, None, :] + dirs[:, None, :] * (t[None, :, None] * L[:, None, None])
 
 
+def proposal_weights(z):
+    """(w [n, nbins], T_end [n]) of the density reading of the logits."""
+    d = torch.nn.functional.softplus(z)
+    S = torch.cumsum(d, dim=1)
+    T = torch.exp(-(S - d))
+    a = -torch.expm1(-d)
+    return T * a, torch.exp(-S[:, -1])
+
+
 def ray_loss(z, q):
-    """sum over rays of -sum_b q_b log softmax(z)_b ; z, q [n_rays, nbins]."""
-    return -(q * torch.log_softmax(z, dim=1)).sum()
+    """sum over rays of -sum_b q_b log w_b - (1 - sum q) log T_end."""
+    d = torch.nn.functional.softplus(z)
+    S = torch.cumsum(d, dim=1)
+    logw = -(S - d) + torch.log(-torch.expm1(-d))
+    return (-(q * logw).sum(1) + (1 - q.sum(1)) * S[:, -1]).sum()
 
 
 def ray_loss_grad(z, q):
-    """Closed form d loss / d z = Q softmax(z) - q."""
-    return q.sum(1, keepdim=True) * torch.softmax(z, dim=1) - q
+    """Closed form dL/dz_b = sigmoid(z_b) [(1 - sum_{c<=b} q_c) - q_b / expm1(d_b)]."""
+    d = torch.nn.functional.softplus(z)
+    return torch.sigmoid(z) * ((1 - torch.cumsum(q, dim=1)) - q / torch.expm1(d))
 
 
 def bin_targets(w, count_render, nbins=NBINS):
@@ -148,7 +170,10 @@ def fine_samples(z, L, K=32, eps=0.05):
     """Inverse-CDF fine sample positions t [n_rays, K] from logits z
     [n_rays, nbins] on [0, L]."""
     n, nb = z.shape
-    pdf = (1 - eps) * torch.softmax(z.to(torch.float64), dim=1) + eps / nb
+    w, _ = proposal_weights(z.to(torch.float64))
+    tot = w.sum(1, keepdim=True)
+    wn = torch.where(tot > 1e-30, w / tot.clamp_min(1e-300), torch.full_like(w, 1.0 / nb))
+    pdf = (1 - eps) * wn + eps / nb
     cdf = torch.cat([torch.zeros((n, 1), dtype=torch.float64), torch.cumsum(pdf, dim=1)], dim=1)
     cdf = cdf / cdf[:, -1:]
     m = K - 2

@@ -237,8 +237,8 @@ __global__ void __launch_bounds__(kMlpBlock) mlp_fwd_kernel(
     }
 }
 
-// Backward, one lane per point and one wavefront per ray: softmax
-// cross-entropy against the bin targets, then dpre3 -> dpre2 -> dpre1 in
+// Backward, one lane per point and one wavefront per ray: the composited
+// log-likelihood against the bin targets, then dpre3 -> dpre2 -> dpre1 in
 // float32 (weights at their bfloat16 values, read from LDS).
 __global__ void __launch_bounds__(256) mlp_bwd_point_kernel(
     const float *__restrict__ logits, const float *__restrict__ target, int64_t n_rays,
@@ -256,22 +256,31 @@ __global__ void __launch_bounds__(256) mlp_bwd_point_kernel(
     const int64_t pt = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t ray = pt / kBins;
     if (ray >= n_rays) return;  // whole wavefronts (one ray = one wave)
+    // density reading of the logits, composited over the ray's 64 bins
+    // (lane = bin): d = softplus(z), a = 1 - e^-d, w = e^{-sum_{c<b} d_c} a;
+    // loss = -sum q log w - (1 - sum q) sum d;
+    // dL/dz_b = sigmoid(z_b) [(1 - sum_{c<=b} q_c) - q_b / expm1(d_b)]
     const float z = logits[pt], q = target[pt];
-    float m = z, Q = q;
+    const int lane = threadIdx.x & (kWave - 1);
+    const float d = z > 15.0f ? z : log1pf(expf(z));
+    float Sd = d, Cq = q;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        m = fmaxf(m, __shfl_xor(m, o, kWave));
-        Q += __shfl_xor(Q, o, kWave);
+    for (int o = 1; o < kWave; o <<= 1) {
+        const float a = __shfl_up(Sd, o, kWave), b = __shfl_up(Cq, o, kWave);
+        if (lane >= o) {
+            Sd += a;
+            Cq += b;
+        }
     }
-    const float e = expf(z - m);
-    float S = e;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) S += __shfl_xor(S, o, kWave);
-    const float dz = Q * (e / S) - q;
-    float l = -q * (z - m - logf(S));
+    const float Stot = __shfl(Sd, kWave - 1, kWave), Qtot = __shfl(Cq, kWave - 1, kWave);
+    float l = 0.0f;
+    if (q > 0.0f) l = -q * (-(Sd - d) + logf(fmaxf(-expm1f(-d), 1e-30f)));
+    if (lane == kWave - 1) l += (1.0f - Qtot) * Stot;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, kWave);
-    if ((threadIdx.x & 63) == 0 && loss) loss[ray] = l;
+    if (lane == 0 && loss) loss[ray] = l;
+    const float sig = 1.0f / (1.0f + expf(-z));
+    const float dz = sig * ((1.0f - Cq) - (q > 0.0f ? q / fmaxf(expm1f(d), 1e-30f) : 0.0f));
     dz_out[pt] = dz;
     // d2[i] = [x2 > 0] sum_o W3[o][i] dpre3[o];  d1 likewise through W2.
     // Loops run over o at runtime and over i fully unrolled, so the register
@@ -384,7 +393,8 @@ __global__ void __launch_bounds__(256) adam_kernel(float *__restrict__ theta,
 }
 
 // Inverse-CDF fine samples, one lane per ray: t = 0, t = L and K-2 interior
-// samples at u = (k + 0.5)/(K-2) of the mixture pdf (1-eps) softmax + eps/64.
+// samples at u = (k + 0.5)/(K-2) of the mixture pdf (1-eps) w/sum(w) + eps/64,
+// w the composited proposal weights of the bins.
 __global__ void __launch_bounds__(256) sampler_kernel(const float *__restrict__ logits,
                                                       const float4 *__restrict__ ray_state,
                                                       int64_t n_rays, float interval, int K,
@@ -394,22 +404,33 @@ __global__ void __launch_bounds__(256) sampler_kernel(const float *__restrict__ 
     const uint32_t count = __float_as_uint(ray_state[2 * r].w);
     const float L = count >= 2 ? (float)(count - 1) * interval : 0.0f;
     const float *z = logits + r * kBins;
-    float m = z[0];
-    for (int b = 1; b < kBins; ++b) m = fmaxf(m, z[b]);
-    float S = 0.0f;
-    for (int b = 0; b < kBins; ++b) S += expf(z[b] - m);
-    const float a = (1.0f - eps) / S, u0 = eps / kBins;
+    // proposal weights w_b = e^{-sum_{c<b} d_c} (1 - e^{-d_b}), d = softplus(z)
+    float tot = 0.0f, S = 0.0f;
+    for (int b = 0; b < kBins; ++b) {
+        const float d = z[b] > 15.0f ? z[b] : log1pf(expf(z[b]));
+        tot += expf(-S) * -expm1f(-d);
+        S += d;
+    }
+    const float u0 = eps / kBins;
+    const float a = tot > 1e-30f ? (1.0f - eps) / tot : 0.0f;
+    const float uni = tot > 1e-30f ? u0 : 1.0f / kBins;
+    auto weight = [&](float Sx, float zb) {
+        const float d = zb > 15.0f ? zb : log1pf(expf(zb));
+        return a * (expf(-Sx) * -expm1f(-d)) + uni;
+    };
     float *out = tlist + r * K;
     out[0] = 0.0f;
     const int nm = K - 2;
     int b = 0;
-    float c_lo = 0.0f, pdf = a * expf(z[0] - m) + u0;
+    float Sb = 0.0f;  // sum of densities before bin b
+    float c_lo = 0.0f, pdf = weight(0.0f, z[0]);
     for (int k = 0; k < nm; ++k) {
         const float u = ((float)k + 0.5f) / (float)nm;
         while (b < kBins - 1 && c_lo + pdf <= u) {
             c_lo += pdf;
+            Sb += z[b] > 15.0f ? z[b] : log1pf(expf(z[b]));
             ++b;
-            pdf = a * expf(z[b] - m) + u0;
+            pdf = weight(Sb, z[b]);
         }
         const float frac = fminf(fmaxf((u - c_lo) / pdf, 0.0f), 1.0f);
         out[1 + k] = ((float)b + frac) * (1.0f / kBins) * L;

@@ -29,12 +29,19 @@ def test_bf16_rounding_is_round_to_nearest_even():
     assert r.tolist() == [1.0, 1.0 + 2 ** -6, 1.0, -3.0]
 
 
-def test_loss_gradient_closed_form():
+def test_loss_gradient_closed_form_and_weights():
     g = torch.Generator().manual_seed(0)
     z = torch.randn((5, 64), generator=g, dtype=torch.float64).requires_grad_(True)
     q = torch.rand((5, 64), generator=g, dtype=torch.float64)
+    q = q / q.sum(1, keepdim=True) * torch.rand((5, 1), generator=g, dtype=torch.float64)
     om.ray_loss(z, q).backward()
     assert torch.allclose(z.grad, om.ray_loss_grad(z.detach(), q))
+    # weights and pass-through mass form a distribution
+    w, T = om.proposal_weights(z.detach())
+    assert torch.allclose(w.sum(1) + T, torch.ones(5, dtype=torch.float64))
+    # the loss is minimised (gradient zero) when w reproduces the target
+    zt = torch.log(torch.expm1(-torch.log1p(-(q / (1 - torch.cat([torch.zeros((5, 1), dtype=torch.float64), torch.cumsum(q, 1)[:, :-1]], 1))))))
+    assert torch.allclose(om.ray_loss_grad(zt, q), torch.zeros_like(q), atol=1e-9)
 
 
 def test_network_finite_differences():
@@ -62,14 +69,16 @@ def test_bin_targets_partition_the_weights():
 
 
 def test_sampler_invariants():
-    z = torch.full((1, 64), 0.0, dtype=torch.float64)
+    z = torch.full((1, 64), -60.0, dtype=torch.float64)
     t = om.fine_samples(z, torch.tensor([2.0]), K=34, eps=0.05)[0]
     assert t[0] == 0 and t[-1] == 2.0
-    # a uniform pdf gives uniformly spaced interior samples
-    assert torch.allclose(t[1:-1], (torch.arange(32, dtype=torch.float64) + 0.5) / 32 * 2.0)
-    # all mass in bin 10 (up to eps): the interior samples concentrate there
+    # a (near-)uniform pdf gives uniformly spaced interior samples
+    assert torch.allclose(t[1:-1], (torch.arange(32, dtype=torch.float64) + 0.5) / 32 * 2.0,
+                          atol=1e-6)
+    # an opaque bin 10 takes the mass, and hides the opaque bin 40 behind it
     z = torch.full((1, 64), -50.0, dtype=torch.float64)
     z[0, 10] = 50.0
+    z[0, 40] = 50.0
     t = om.fine_samples(z, torch.tensor([64.0]), K=32, eps=0.05)[0]
     inside = ((t >= 10) & (t < 11)).sum()
     assert int(inside) >= 27 and torch.all(t[1:] > t[:-1])
