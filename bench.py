@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--cpu-rays", type=int, default=196608,
                     help="rays in the CPU oracle sample (0: skip the CPU baseline)")
     ap.add_argument("--scatter", type=int, default=0, help="fused scatter mode (0 tile, 1 atomic)")
+    ap.add_argument("--proposal-s", type=str, default="40,5",
+                    help="logistic coefficients at which the proposal path is measured "
+                         "(comma list; empty: skip)")
+    ap.add_argument("--proposal-train-steps", type=int, default=1000)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run a plain device copy of this many bytes first (a counter "
                          "calibration kernel of known traffic for rocprofv3 --pmc runs)")
@@ -60,9 +64,9 @@ def algorithmic_bytes(R, C, P):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the path "
-                                   "shards by pose with one gradient all-reduce per step, "
-                                   "described in DESIGN.md)"}))
+        print(json.dumps({"error": "only one GPU is measured here; --gpus must be 1 (the path "
+                                   "would shard by pose with replicated grids and one gradient "
+                                   "all-reduce per step)"}))
         sys.exit(2)
     import numpy as np
     import torch
@@ -159,6 +163,13 @@ def main():
         del G2, D2, cam_objs
         torch.cuda.empty_cache()
 
+    # ---------------------------------------------------- proposal path ----
+    prop_results = []
+    for s_txt in [x for x in args.proposal_s.split(",") if x.strip()]:
+        prop_results.append(measure_proposal(w, grid.detach(), data.detach(), cams, cam_buf,
+                                             targets, float(s_txt), args))
+        log(f"proposal: {prop_results[-1]}")
+
     # ------------------------------------------------------- roofline ----
     dom = max(kern_ms, key=kern_ms.get)
     byts = algorithmic_bytes(w.R, 32, P) if dom == "fused_backward" else (
@@ -184,9 +195,78 @@ def main():
                       "poses": w.n_poses, "logistic_coef": w.s, "path": "fused plain",
                       "scatter_mode": args.scatter},
            "op_surface_rays_per_s": round(op_rays, 1) if op_rays else None,
+           "proposal": prop_results,
            "kernel_ms": {k: round(v, 4) for k, v in kern_ms.items()},
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
+
+
+def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps=None):
+    """Proposal-guided vs plain at logistic coefficient s on the same scene:
+    train the proposal on poses 0..89, then time plain and proposal
+    forward+backward steps on poses 90.. and compare PSNR on poses 95..99
+    against a dense plain render with 8x the samples (GT)."""
+    import torch
+
+    from qsdvr_amd import fused, proposal
+
+    P = w.width * w.height
+    steps = steps or args.steps
+    prop = proposal.Proposal(seed=0)
+    t0 = time.perf_counter()
+    proposal.train(prop, grid, data, cams[:90], w.interval, s,
+                   steps=args.proposal_train_steps, batch=8192, lr=3e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    G = grid.clone().requires_grad_(True)
+    D = data.clone().requires_grad_(True)
+    CB = 80
+
+    def plain(p):
+        G.grad = None
+        D.grad = None
+        c, _ = fused.FusedRender.apply(G, D, cam_buf[p * CB:(p + 1) * CB], 1, w.width, w.height,
+                                       None, w.interval, s, args.scatter)
+        ((c - targets[p]) ** 2).mean().backward()
+
+    def guided(p):
+        G.grad = None
+        D.grad = None
+        c, _ = proposal.render_proposal(G, D, cam_buf[p * CB:(p + 1) * CB], 1, w.width, w.height,
+                                        w.interval, s, prop, K=K)
+        ((c - targets[p]) ** 2).mean().backward()
+
+    out = {"logistic_coef": s, "fine_samples": K, "coarse_bins": 64,
+           "train_steps": args.proposal_train_steps, "train_seconds": round(train_s, 2)}
+    for name, fn in (("plain", plain), ("proposal", guided)):
+        for i in range(2):
+            fn(90 + i)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for i in range(steps):
+            fn(90 + (i % 10))
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        out[name + "_ms_per_step"] = round(ms, 3)
+        out[name + "_rays_per_s"] = round(P / (ms / 1e3), 1)
+    out["speedup"] = round(out["plain_ms_per_step"] / out["proposal_ms_per_step"], 3)
+    with torch.no_grad():
+        test = cams[95:100]
+        tb = fused.camera_buffer(test)
+        gt, _ = fused.render(grid, data, test, w.interval / 8, s)
+        pl, _ = fused.render(grid, data, test, w.interval, s)
+        pr, _ = proposal.render_proposal(grid, data, tb, len(test), w.width, w.height, w.interval,
+                                         s, prop, K=K)
+        out["psnr_plain_vs_gt_db"] = round(proposal.psnr(pl, gt), 3)
+        out["psnr_proposal_vs_gt_db"] = round(proposal.psnr(pr, gt), 3)
+        out["psnr_loss_db"] = round(out["psnr_plain_vs_gt_db"] - out["psnr_proposal_vs_gt_db"], 3)
+        out["psnr_proposal_vs_plain_db"] = round(proposal.psnr(pr, pl), 3)
+        out["gt"] = f"plain renderer at 8x samples ({8 * w.n_samples} per diagonal), poses 95-99"
+    del G, D
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(w, cam, grid, data, target, n_rays):

@@ -392,50 +392,56 @@ __global__ void __launch_bounds__(256) adam_kernel(float *__restrict__ theta,
     theta[id] -= lr * (mm * c1) / (sqrtf(vv * c2) + eps);
 }
 
-// Inverse-CDF fine samples, one lane per ray: t = 0, t = L and K-2 interior
-// samples at u = (k + 0.5)/(K-2) of the mixture pdf (1-eps) w/sum(w) + eps/64,
-// w the composited proposal weights of the bins.
+// Inverse-CDF fine samples, one wavefront per ray (lane = bin): t = 0, t = L
+// and K-2 interior samples at u = (k + 0.5)/(K-2) of the mixture pdf
+// (1-eps) w/sum(w) + eps/64, w the composited proposal weights of the bins.
+__device__ __forceinline__ float wave_scan_incl(float v, int lane) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const float u = __shfl_up(v, o, kWave);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
 __global__ void __launch_bounds__(256) sampler_kernel(const float *__restrict__ logits,
                                                       const float4 *__restrict__ ray_state,
                                                       int64_t n_rays, float interval, int K,
                                                       float eps, float *__restrict__ tlist) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_rays) return;
+    __shared__ float s_lo[4][kBins], s_pdf[4][kBins];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int64_t r = (int64_t)blockIdx.x * 4 + wv;
+    if (r >= n_rays) return;  // whole wavefront
     const uint32_t count = __float_as_uint(ray_state[2 * r].w);
     const float L = count >= 2 ? (float)(count - 1) * interval : 0.0f;
-    const float *z = logits + r * kBins;
-    // proposal weights w_b = e^{-sum_{c<b} d_c} (1 - e^{-d_b}), d = softplus(z)
-    float tot = 0.0f, S = 0.0f;
-    for (int b = 0; b < kBins; ++b) {
-        const float d = z[b] > 15.0f ? z[b] : log1pf(expf(z[b]));
-        tot += expf(-S) * -expm1f(-d);
-        S += d;
-    }
-    const float u0 = eps / kBins;
-    const float a = tot > 1e-30f ? (1.0f - eps) / tot : 0.0f;
-    const float uni = tot > 1e-30f ? u0 : 1.0f / kBins;
-    auto weight = [&](float Sx, float zb) {
-        const float d = zb > 15.0f ? zb : log1pf(expf(zb));
-        return a * (expf(-Sx) * -expm1f(-d)) + uni;
-    };
+    const float z = logits[r * kBins + lane];
+    const float d = z > 15.0f ? z : log1pf(expf(z));
+    const float S = wave_scan_incl(d, lane);
+    const float wb = expf(-(S - d)) * -expm1f(-d);
+    float tot = wb;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, kWave);
+    const float pdf = tot > 1e-30f ? (1.0f - eps) * wb / tot + eps / kBins : 1.0f / kBins;
+    const float hi = wave_scan_incl(pdf, lane);
+    s_lo[wv][lane] = hi - pdf;
+    s_pdf[wv][lane] = pdf;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
     float *out = tlist + r * K;
-    out[0] = 0.0f;
     const int nm = K - 2;
-    int b = 0;
-    float Sb = 0.0f;  // sum of densities before bin b
-    float c_lo = 0.0f, pdf = weight(0.0f, z[0]);
-    for (int k = 0; k < nm; ++k) {
+    for (int k = lane; k < nm; k += kWave) {
         const float u = ((float)k + 0.5f) / (float)nm;
-        while (b < kBins - 1 && c_lo + pdf <= u) {
-            c_lo += pdf;
-            Sb += z[b] > 15.0f ? z[b] : log1pf(expf(z[b]));
-            ++b;
-            pdf = weight(Sb, z[b]);
-        }
-        const float frac = fminf(fmaxf((u - c_lo) / pdf, 0.0f), 1.0f);
-        out[1 + k] = ((float)b + frac) * (1.0f / kBins) * L;
+        int lo = 0;  // last bin with cdf_lo <= u
+#pragma unroll
+        for (int step = kBins / 2; step > 0; step >>= 1)
+            if (lo + step < kBins && s_lo[wv][lo + step] <= u) lo += step;
+        const float frac = fminf(fmaxf((u - s_lo[wv][lo]) / s_pdf[wv][lo], 0.0f), 1.0f);
+        out[1 + k] = ((float)lo + frac) * (1.0f / kBins) * L;
     }
-    out[K - 1] = L;
+    if (lane == 0) {
+        out[0] = 0.0f;
+        out[K - 1] = L;
+    }
 }
 
 }  // namespace qsdvr
@@ -540,7 +546,7 @@ int qsdvr_proposal_sample(const float *logits, const float *ray_state, int64_t n
                   "qsdvr_proposal_sample: bad sizes");
     if (n_rays == 0) return QSDVR_OK;
     QSDVR_REQUIRE(logits && ray_state && tlist, "qsdvr_proposal_sample: null pointer argument");
-    sampler_kernel<<<grid_1d(n_rays, 256), 256, 0, as_stream(stream)>>>(
+    sampler_kernel<<<(unsigned)((n_rays + 3) / 4), 256, 0, as_stream(stream)>>>(
         logits, reinterpret_cast<const float4 *>(ray_state), n_rays, interval, K, eps, tlist);
     QSDVR_LAUNCHED();
     return QSDVR_OK;
