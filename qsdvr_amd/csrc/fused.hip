@@ -332,23 +332,23 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_kernel(
 
 // ------------------------------------------------ tile-reduced adjoint ----
 // One wavefront per block.  At every step j the 64 rays of the tile touch a
-// small box of cells (typically 2x2x2 to 3x3x3).  Their appearance
-// gradients are summed in a per-wave LDS box accumulator and then flushed
-// with one coalesced global atomic per touched (vertex, channel), line by
-// line of the box (each (z, y) line is contiguous in LDS and in HBM).
-//
-// Conflict-free accumulation: every lane parks its gradient row in its own
-// LDS staging row and, at iteration c, adds channel (c + l) mod C of it
-// (l = lane) to its 8 vertices with a plain read-modify-write.  Within one
-// half-wave the 32 lanes address 32 different channels -- never the same
-// LDS word even when they share a vertex -- so the two half-waves simply take
-// turns.  (This is about 4x faster than LDS float atomics on gfx950, and
-// about 9x faster than one global atomic per sample and channel.)  The 13
+// small box of cells (typically 2x2x2 to 3x3x3).  The appearance gradient
+// of the step is a small dense product: box[v][ch] = sum over lanes l of
+// W[l][v] * G[l][ch], where G[l] is lane l's gradient of its interpolated
+// appearance row and W[l][v] its trilinear weight of box vertex v (8
+// non-zeros per lane).  G and W are staged transposed in LDS and multiplied
+// on the matrix cores in float32 (v_mfma_f32_16x16x4_f32); the result tile
+// is flushed straight from the accumulators with one global atomic per
+// touched (vertex, channel), 16 consecutive channels of 4 vertices per
+// instruction.  (The previous scheme -- per-lane LDS read-modify-write of
+// 8 x C words per sample -- spent most of its time in LDS traffic; one
+// global atomic per sample and channel was ~40x slower still.)  The 13
 // quadric gradients per sample go through LDS atomics on a cell-box
 // accumulator.  Steps whose box exceeds kCapV vertices (about 4% of the
 // uniform steps at 800x800, 12% of proposal-guided steps, mostly silhouette
-// tiles) are split into rounds of 3x3x3-cell boxes.
+// tiles) are split into rounds of 3x3x3-cell boxes (4x4x4 = 64 vertices).
 constexpr int kCapV = 64;
+static_assert(kCapV >= 64, "a fallback round's 3x3x3-cell box has 64 vertices");
 
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
@@ -366,16 +366,31 @@ __device__ __forceinline__ void lds_add(float *p, float v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-template <int C>
-__device__ __forceinline__ int rotate_channel(int c, int lane) {
-    int ch = c + lane;
-    if constexpr ((C & (C - 1)) == 0) {
-        return ch & (C - 1);
-    } else {
-        if (ch >= C) ch -= C;
-        if (ch >= C) ch -= C;
-        return ch;
-    }
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Four K steps of a 16x16 float32 tile product: lane t supplies
+// A[t % 16][t / 16] and B[t / 16][t % 16] of each 16x16x4 step and holds
+// D[4 (t / 16) + i][t % 16] for i < 4.
+__device__ __forceinline__ floatx4 mfma4(float4 a, float4 b, floatx4 d) {
+    d = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, d, 0, 0, 0);
+    return d;
+}
+
+// LDS layout of the transposed operands.  Gradient rows: [channel][64
+// lanes + 4 pad] (row stride a multiple of 4 floats, so rows stay float4
+// aligned and 16 consecutive rows start on 16 different bank groups).
+// Weights: [vertex][64 lanes] with the float4 column groups XOR-swizzled by
+// the row, element (row, col) at row * 64 + (col ^ 4 (row % 16)), which is
+// as conflict-free as the padding for the fragment reads and needs no pad.
+constexpr int kGradStride = kWave + 4;
+__device__ __forceinline__ int swz(int row, int col) { return row * kWave + (col ^ ((row & 15) << 2)); }
+
+template <int NC>
+constexpr int tile_channels() {
+    return (3 * NC + 5 + 15) / 16 * 16;  // appearance channels padded to 16-wide tiles
 }
 
 template <int NC>
@@ -388,15 +403,16 @@ __global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
     float *__restrict__ gq, float *__restrict__ gdata, const float4 *__restrict__ ray_state,
     const float *__restrict__ tlist, int K) {
     constexpr int C = 3 * NC + 5;
-    constexpr int CS = (C + 31) / 32 * 32;  // LDS row stride: a multiple of the 32 banks
-    extern __shared__ float lds[];
-    float *stage = lds;                 // [64][CS]: per-lane gradient rows
-    float *acc = lds + kWave * CS;      // [kCapV][CS]: vertex-box accumulator
-    float *qacc = acc + kCapV * CS;     // [kCapV][8]: cell-box accumulator (A..G, pad)
-    float *qrow = qacc + kCapV * 8 + threadIdx.x * 16;
+    constexpr int CP = tile_channels<NC>();
+    extern __shared__ float4 lds4[];
+    float *gt = reinterpret_cast<float *>(lds4);  // [CP][68]: gradient rows, channel-major
+    float *wt = gt + CP * kGradStride;            // [kCapV][64]: trilinear weights, vertex-major
+    float *qacc = wt + kCapV * kWave;               // [kCapV][8]: cell-box accumulator (A..G, pad)
+    float *qrow = qacc + kCapV * 8 + threadIdx.x * 16;  // [64][16]: per-lane quadric gradients
     const int lane = threadIdx.x;
-    for (int i = lane; i < kCapV * CS; i += kWave) acc[i] = 0.0f;
-    for (int i = lane; i < kCapV * 8; i += kWave) qacc[i] = 0.0f;
+    // float4 slots of this lane's four K groups in a swizzled row
+    const int k0 = (4 * (lane >> 4)) ^ (lane & 15), k1 = k0 ^ 1, k2 = k0 ^ 2, k3 = k0 ^ 3;
+    for (int i = lane; i < CP * kGradStride + kCapV * kWave + kCapV * 8; i += kWave) gt[i] = 0.0f;
 
     RaySetup ray = setup_ray<1>(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state, tlist, K);
     const bool live = ray.out >= 0 && ray.renders > 0;
@@ -472,21 +488,14 @@ __global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
             cp0 = cp1;
         }
 
-        // stage this step's gradient rows once
-        {
-            float *row = stage + lane * CS;
+        // stage this step's gradient row once, transposed: gt[c][lane]
 #pragma unroll
-            for (int c = 0; c < C; ++c) row[c] = dd[c];
+        for (int c = 0; c < C; ++c) gt[c * kGradStride + lane] = dd[c];
 #pragma unroll
-            for (int k = 0; k < 7; ++k) {
-                qrow[k] = gs7[k];
-                qrow[8 + k] = gn7[k];
-            }
-            qrow[7] = 0.0f;
-            qrow[15] = 0.0f;
+        for (int k = 0; k < 7; ++k) {
+            qrow[k] = gs7[k];
+            qrow[8 + k] = gn7[k];
         }
-        __syncthreads();
-        const float *row = stage + lane * CS;
         bool pend_r = act_r, pend_s = act_s;  // parts not yet accumulated
         // Rounds: the tight box of all pending cells when it fits in kCapV
         // vertices (the common case: one round); otherwise the 3x3x3-cell
@@ -522,26 +531,17 @@ __global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
             const bool in_s = pend_s && sc.ix >= lx && sc.ix <= hx && sc.iy >= ly &&
                               sc.iy <= hy && sc.iz >= lz && sc.iz <= hz;
             const int vb = (rp.ix - lx) + (rp.iy - ly) * ex + (rp.iz - lz) * sxy;
-            int base[8];
+            const int nv = sxy * ez;
+            int wofs[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k)
-                base[k] = (vb + (k & 1) + ((k >> 1) & 1) * ex + (k >> 2) * sxy) * CS;
-            for (int half = 0; half < 2; ++half) {
-                const bool mine = in_r && ((lane >> 5) == half);
-                for (int c = 0; c < C; ++c) {
-                    if (mine) {
-                        const int ch = rotate_channel<C>(c, lane);
-                        const float v = row[ch];
-                        float o[8];
+                wofs[k] = swz(vb + (k & 1) + ((k >> 1) & 1) * ex + (k >> 2) * sxy, lane);
+            if (in_r) {
 #pragma unroll
-                        for (int k = 0; k < 8; ++k) o[k] = acc[base[k] + ch];
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) acc[base[k] + ch] = o[k] + w[k] * v;
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                }
+                for (int k = 0; k < 8; ++k) wt[wofs[k]] = w[k];
             }
+            // the 13 quadric gradients: LDS atomics on the cell box, 8
+            // lanes of a cell spread over its 7 words
             if (in_s) {
                 const int cb = ((sc.ix - lx) + (sc.iy - ly) * ex + (sc.iz - lz) * sxy) * 8;
 #pragma unroll
@@ -558,35 +558,68 @@ __global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
                     if (kk < 6) lds_add(qacc + cb + kk, qrow[8 + kk]);
                 }
             }
+            // box-vertex offsets in the global grids, vertex v held by lane v
+            int vofs = 0;
+            if (lane < nv) {
+                const int vz = lane / sxy, r2 = lane - vz * sxy, vy = r2 / ex, vx = r2 - vy * ex;
+                vofs = vx + vy * R + vz * (int)RR;
+            }
+            const int64_t vbase = (int64_t)lx + (int64_t)ly * R + (int64_t)lz * RR;
             __syncthreads();
-            // flush line by line: (sz, sy) rows of the box are contiguous both
-            // in the LDS accumulators and in the global grids
-            for (int sz = 0; sz < ez; ++sz) {
-                for (int sy = 0; sy < ey; ++sy) {
-                    const int sbase = sy * ex + sz * sxy;
-                    const int64_t vrow0 =
-                        (int64_t)lx + (int64_t)(ly + sy) * R + (int64_t)(lz + sz) * RR;
-                    for (int i = lane; i < ex * C; i += kWave) {
-                        const int sx = i / C, ch = i - sx * C;
-                        float *a = acc + (sbase + sx) * CS + ch;
-                        const float v = *a;
-                        if (v != 0.0f) {
-                            *a = 0.0f;
-                            atomic_addf(gdata + (vrow0 + sx) * C + ch, v);
-                        }
-                    }
-                    if (sz < ez - 1 && sy < ey - 1) {
-                        for (int i = lane; i < (ex - 1) * 7; i += kWave) {
-                            const int sx = i / 7, k = i - sx * 7;
-                            float *a = qacc + (sbase + sx) * 8 + k;
-                            const float v = *a;
-                            if (v != 0.0f) {
-                                *a = 0.0f;
-                                atomic_addf(gq + (vrow0 + sx) * 7 + k, v);
-                            }
+            // Appearance: box[v][ch] = sum over lanes l of wt[v][l] * gt[ch][l],
+            // a (box vertices x 64) x (64 x C) product on the matrix cores
+            // (v_mfma_f32_16x16x4_f32, float32 throughout).  K block kb of
+            // the 16x16x4 tile takes lanes {kb, 16 + kb, 32 + kb, 48 + kb},
+            // so every lane reads its 16 operands of a tile as 4 float4.
+            if (__ballot(in_r) != 0) {
+                const int g = lane >> 4, m = lane & 15;
+                const int nmb = (nv + 15) >> 4;
+#pragma unroll
+                for (int nb = 0; nb < CP / 16; ++nb) {
+                    const float4 *sb =
+                        reinterpret_cast<const float4 *>(gt + (16 * nb + m) * kGradStride + 16 * g);
+                    const float4 b0 = sb[0], b1 = sb[1], b2 = sb[2], b3 = sb[3];
+                    const int ch = 16 * nb + m;
+                    for (int mb = 0; mb < nmb; ++mb) {
+                        const float4 *sa =
+                            reinterpret_cast<const float4 *>(wt + (16 * mb + m) * kWave);
+                        const float4 a0 = sa[k0], a1 = sa[k1], a2 = sa[k2], a3 = sa[k3];
+                        floatx4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+                        d = mfma4(a0, b0, d);
+                        d = mfma4(a1, b1, d);
+                        d = mfma4(a2, b2, d);
+                        d = mfma4(a3, b3, d);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int v = 16 * mb + 4 * g + i;
+                            const int vo = __shfl(vofs, v, kWave);
+                            if (v < nv && ch < C && d[i] != 0.0f)
+                                atomic_addf(gdata + (vbase + vo) * C + ch, d[i]);
                         }
                     }
                 }
+            }
+            // flush the cell box line by line: (sz, sy) lines of cells are
+            // contiguous both in LDS and in the quadric grid
+            for (int sz = 0; sz + 1 < ez; ++sz) {
+                for (int sy = 0; sy + 1 < ey; ++sy) {
+                    const int sbase = sy * ex + sz * sxy;
+                    const int64_t vrow0 = vbase + (int64_t)sy * R + (int64_t)sz * RR;
+                    for (int i = lane; i < (ex - 1) * 7; i += kWave) {
+                        const int sx = i / 7, k = i - sx * 7;
+                        float *a = qacc + (sbase + sx) * 8 + k;
+                        const float v = *a;
+                        if (v != 0.0f) {
+                            *a = 0.0f;
+                            atomic_addf(gq + (vrow0 + sx) * 7 + k, v);
+                        }
+                    }
+                }
+            }
+            // restore the all-zero weight matrix for the next round / step
+            if (in_r) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) wt[wofs[k]] = 0.0f;
             }
             __syncthreads();
             pend_r = pend_r && !in_r;
@@ -597,9 +630,8 @@ __global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
 
 template <int NC>
 constexpr size_t tile_lds_bytes() {
-    constexpr int C = 3 * NC + 5;
-    constexpr int CS = (C + 31) / 32 * 32;
-    return (size_t)(kWave * CS + kCapV * CS + kCapV * 8 + kWave * 16) * sizeof(float);
+    return (size_t)(tile_channels<NC>() * kGradStride + kCapV * kWave + kCapV * 8 + kWave * 16) *
+           sizeof(float);
 }
 
 static int64_t fused_waves(uint32_t W, uint32_t H, int n_cams, const int32_t *ray_pixel,
