@@ -195,7 +195,7 @@ __device__ __forceinline__ float sdf_at(const RaySetup &ray, uint32_t j, float i
     return quadric_sdf(load_quadric(qgrid, cp.cell), cp.local, inv_reso);
 }
 
-template <int NC>
+template <int NC, bool BINS>
 __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
     const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
     const int32_t *__restrict__ ray_pixel,
@@ -213,6 +213,11 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
         CellPoint cp;
         float e0 = expf(-s * sdf_at(ray, 0, interval, half_reso, R, inv_reso, qgrid, cp));
         float T = 1.0f;
+        // 64 equal bins over the ray's sampled segment (training targets).
+        // The bin index never decreases along the ray, so each bin is summed
+        // in a register and stored once (the caller zeroes the bins).
+        int bin = 0;
+        float bin_sum = 0.0f;
         for (uint32_t j = 0; j < ray.renders; ++j) {
             float e1 = expf(-s * sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp));
             float a = fmaxf((e1 - e0) / (1.0f + e1), 0.0f);
@@ -224,14 +229,19 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
             col.y += ev.rgb.y * wt;
             col.z += ev.rgb.z * wt;
             dep += ev.t * wt;
-            if (bins) {
-                // 64 equal bins over the ray's sampled segment (training targets)
-                int b = (int)(((float)j + 0.5f) / (float)ray.renders * 64.0f);
-                bins[ray.out * 64 + min(b, 63)] += wt;
+            if constexpr (BINS) {
+                const int b = min((int)(((float)j + 0.5f) / (float)ray.renders * 64.0f), 63);
+                if (b != bin) {
+                    bins[ray.out * 64 + bin] = bin_sum;
+                    bin = b;
+                    bin_sum = 0.0f;
+                }
+                bin_sum += wt;
             }
             T = T * (1.0f - a);
             e0 = e1;
         }
+        if constexpr (BINS) bins[ray.out * 64 + bin] = bin_sum;
     }
     store3(color, ray.out, col);
     if (depth) depth[ray.out] = dep;
@@ -343,8 +353,12 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_kernel(
 // instruction.  (The previous scheme -- per-lane LDS read-modify-write of
 // 8 x C words per sample -- spent most of its time in LDS traffic; one
 // global atomic per sample and channel was ~40x slower still.)  The 13
-// quadric gradients per sample go through LDS atomics on a cell-box
-// accumulator.  Steps whose box exceeds kCapV vertices (about 4% of the
+// quadric gradients per sample (7 at the sdf sample's cell, 6 at the render
+// sample's) are summed over runs of equal cells along each 8-lane pixel row
+// with DPP lane shifts, and each run adds its sums with global atomics from
+// its last lane: the rays of a tile share a handful of cells, so LDS
+// atomics on a cell box serialised on the same words.  Steps whose
+// appearance box exceeds kCapV vertices (about 4% of the
 // uniform steps at 800x800, 12% of proposal-guided steps, mostly silhouette
 // tiles) are split into rounds of 3x3x3-cell boxes (4x4x4 = 64 vertices).
 constexpr int kCapV = 64;
@@ -362,9 +376,59 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-__device__ __forceinline__ void lds_add(float *p, float v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+// DPP lane shifts within rows of 16 lanes (row_shr:D = 0x110 + D, row_shl:1
+// = 0x101); a lane whose source lies outside its row reads 0
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
 }
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+
+// One step of a segmented inclusive scan over the 8-lane row: lane x adds
+// the partial sum of lane x - D unless a run starts in (x - D, x].
+template <int N, int D>
+__device__ __forceinline__ void run_scan_step(float v[7], bool &f, int x) {
+    float p[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) p[k] = dpp_f<0x110 + D>(v[k]);
+    const bool pf = dpp_i<0x110 + D>((int)f) != 0;
+    if (x >= D && !f) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] += p[k];
+        f = pf;
+    }
+}
+
+// Adds the first N of v[] to quadric cell `key` (key < 0: nothing), summed
+// first over runs of equal keys along each 8-lane pixel row; the run's last
+// lane issues the global atomics.  Must be called by all 64 lanes.
+template <int N>
+__device__ __forceinline__ void row_run_scatter(float v[7], int key, int lane,
+                                                float *__restrict__ gq) {
+    const int x = lane & 7;
+    const int left = dpp_i<0x111>(key);
+    const bool head = x == 0 || key < 0 || left != key;
+    const bool next_head = dpp_i<0x101>((int)head) != 0;
+    bool f = head;
+    run_scan_step<N, 1>(v, f, x);
+    run_scan_step<N, 2>(v, f, x);
+    run_scan_step<N, 4>(v, f, x);
+    if (key >= 0 && (x == 7 || next_head)) {
+        float *dst = gq + 7 * (int64_t)key;
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            if (v[k] != 0.0f) atomic_addf(dst + k, v[k]);
+    }
+}
+
+// Orders this wave's LDS accesses for the compiler only.  The tile kernels
+// run one wave per workgroup, and a wave's LDS instructions execute in
+// issue order, so LDS data written by one lane is visible to a later read
+// by another lane of the wave without s_barrier.
+__device__ __forceinline__ void wave_lds_order() { __asm__ volatile("" ::: "memory"); }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -382,19 +446,26 @@ __device__ __forceinline__ floatx4 mfma4(float4 a, float4 b, floatx4 d) {
 // LDS layout of the transposed operands.  Gradient rows: [channel][64
 // lanes + 4 pad] (row stride a multiple of 4 floats, so rows stay float4
 // aligned and 16 consecutive rows start on 16 different bank groups).
-// Weights: [vertex][64 lanes] with the float4 column groups XOR-swizzled by
-// the row, element (row, col) at row * 64 + (col ^ 4 (row % 16)), which is
-// as conflict-free as the padding for the fragment reads and needs no pad.
+// Weights: [vertex][32 lanes of one half-wave] with the float4 column groups
+// XOR-swizzled by the row, element (row, col) at
+// row * 32 + (col ^ 4 ((row / 2) % 8)), which keeps the fragment reads of 16
+// consecutive rows on 16 different bank groups with no padding.
 constexpr int kGradStride = kWave + 4;
-__device__ __forceinline__ int swz(int row, int col) { return row * kWave + (col ^ ((row & 15) << 2)); }
+constexpr int kHalf = kWave / 2;
+__device__ __forceinline__ int swz(int row, int col) {
+    return row * kHalf + (col ^ (((row >> 1) & 7) << 2));
+}
 
 template <int NC>
 constexpr int tile_channels() {
     return (3 * NC + 5 + 15) / 16 * 16;  // appearance channels padded to 16-wide tiles
 }
 
+// At most 256 registers per lane (2 waves per SIMD, 8 per CU, which the
+// 16.5 KB of LDS per wave also allows): occupancy hides the latency of the
+// per-sample gathers.
 template <int NC>
-__global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
+__global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
     const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
     const int32_t *__restrict__ ray_pixel,
     int64_t n_rays, const float *__restrict__ qgrid, const float *__restrict__ data, int R,
@@ -406,13 +477,9 @@ __global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
     constexpr int CP = tile_channels<NC>();
     extern __shared__ float4 lds4[];
     float *gt = reinterpret_cast<float *>(lds4);  // [CP][68]: gradient rows, channel-major
-    float *wt = gt + CP * kGradStride;            // [kCapV][64]: trilinear weights, vertex-major
-    float *qacc = wt + kCapV * kWave;               // [kCapV][8]: cell-box accumulator (A..G, pad)
-    float *qrow = qacc + kCapV * 8 + threadIdx.x * 16;  // [64][16]: per-lane quadric gradients
+    float *wt = gt + CP * kGradStride;            // [kCapV][32]: trilinear weights, vertex-major
     const int lane = threadIdx.x;
-    // float4 slots of this lane's four K groups in a swizzled row
-    const int k0 = (4 * (lane >> 4)) ^ (lane & 15), k1 = k0 ^ 1, k2 = k0 ^ 2, k3 = k0 ^ 3;
-    for (int i = lane; i < CP * kGradStride + kCapV * kWave + kCapV * 8; i += kWave) gt[i] = 0.0f;
+    for (int i = lane; i < CP * kGradStride + kCapV * kHalf; i += kWave) gt[i] = 0.0f;
 
     RaySetup ray = setup_ray<1>(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state, tlist, K);
     const bool live = ray.out >= 0 && ray.renders > 0;
@@ -491,12 +558,10 @@ __global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
         // stage this step's gradient row once, transposed: gt[c][lane]
 #pragma unroll
         for (int c = 0; c < C; ++c) gt[c * kGradStride + lane] = dd[c];
-#pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            qrow[k] = gs7[k];
-            qrow[8 + k] = gn7[k];
-        }
-        bool pend_r = act_r, pend_s = act_s;  // parts not yet accumulated
+        // quadric gradients of the sdf sample and of the render sample
+        row_run_scatter<7>(gs7, act_s ? (int)sc.cell : -1, lane, gq);
+        row_run_scatter<6>(gn7, act_r ? (int)rp.cell : -1, lane, gq);
+        bool pend_r = act_r;  // appearance gradient not yet accumulated
         // Rounds: the tight box of all pending cells when it fits in kCapV
         // vertices (the common case: one round); otherwise the 3x3x3-cell
         // box centred on the first pending lane's cell.  Every round takes
@@ -507,19 +572,15 @@ __global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
                 lx = rp.ix; ly = rp.iy; lz = rp.iz;
                 hx = rp.ix; hy = rp.iy; hz = rp.iz;
             }
-            if (pend_s) {
-                lx = min(lx, sc.ix); ly = min(ly, sc.iy); lz = min(lz, sc.iz);
-                hx = max(hx, sc.ix); hy = max(hy, sc.iy); hz = max(hz, sc.iz);
-            }
             lx = wave_min(lx); ly = wave_min(ly); lz = wave_min(lz);
             hx = wave_max(hx); hy = wave_max(hy); hz = wave_max(hz);
             if (hx < lx) break;  // nothing pending (wave-uniform)
             if ((int64_t)(hx - lx + 2) * (hy - ly + 2) * (hz - lz + 2) > kCapV) {
-                const unsigned long long m = __ballot(pend_r || pend_s);
+                const unsigned long long m = __ballot(pend_r);
                 const int leader = __ffsll((long long)m) - 1;
-                const int ax = __shfl(pend_r ? rp.ix : sc.ix, leader, kWave);
-                const int ay = __shfl(pend_r ? rp.iy : sc.iy, leader, kWave);
-                const int az = __shfl(pend_r ? rp.iz : sc.iz, leader, kWave);
+                const int ax = __shfl(rp.ix, leader, kWave);
+                const int ay = __shfl(rp.iy, leader, kWave);
+                const int az = __shfl(rp.iz, leader, kWave);
                 lx = ax - 1; hx = ax + 1;
                 ly = ay - 1; hy = ay + 1;
                 lz = az - 1; hz = az + 1;
@@ -528,36 +589,12 @@ __global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
             const int sxy = ex * ey;
             const bool in_r = pend_r && rp.ix >= lx && rp.ix <= hx && rp.iy >= ly &&
                               rp.iy <= hy && rp.iz >= lz && rp.iz <= hz;
-            const bool in_s = pend_s && sc.ix >= lx && sc.ix <= hx && sc.iy >= ly &&
-                              sc.iy <= hy && sc.iz >= lz && sc.iz <= hz;
             const int vb = (rp.ix - lx) + (rp.iy - ly) * ex + (rp.iz - lz) * sxy;
             const int nv = sxy * ez;
             int wofs[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k)
-                wofs[k] = swz(vb + (k & 1) + ((k >> 1) & 1) * ex + (k >> 2) * sxy, lane);
-            if (in_r) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) wt[wofs[k]] = w[k];
-            }
-            // the 13 quadric gradients: LDS atomics on the cell box, 8
-            // lanes of a cell spread over its 7 words
-            if (in_s) {
-                const int cb = ((sc.ix - lx) + (sc.iy - ly) * ex + (sc.iz - lz) * sxy) * 8;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int kk = (k + lane) & 7;
-                    if (kk < 7) lds_add(qacc + cb + kk, qrow[kk]);
-                }
-            }
-            if (in_r) {
-                const int cb = ((rp.ix - lx) + (rp.iy - ly) * ex + (rp.iz - lz) * sxy) * 8;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int kk = (k + lane) & 7;
-                    if (kk < 6) lds_add(qacc + cb + kk, qrow[8 + kk]);
-                }
-            }
+                wofs[k] = swz(vb + (k & 1) + ((k >> 1) & 1) * ex + (k >> 2) * sxy, lane & (kHalf - 1));
             // box-vertex offsets in the global grids, vertex v held by lane v
             int vofs = 0;
             if (lane < nv) {
@@ -565,73 +602,79 @@ __global__ void __launch_bounds__(kWave) fused_bwd_tile_kernel(
                 vofs = vx + vy * R + vz * (int)RR;
             }
             const int64_t vbase = (int64_t)lx + (int64_t)ly * R + (int64_t)lz * RR;
-            __syncthreads();
-            // Appearance: box[v][ch] = sum over lanes l of wt[v][l] * gt[ch][l],
+            wave_lds_order();
+            // Appearance: box[v][ch] = sum over lanes l of W[l][v] * G[l][ch],
             // a (box vertices x 64) x (64 x C) product on the matrix cores
-            // (v_mfma_f32_16x16x4_f32, float32 throughout).  K block kb of
-            // the 16x16x4 tile takes lanes {kb, 16 + kb, 32 + kb, 48 + kb},
-            // so every lane reads its 16 operands of a tile as 4 float4.
+            // (v_mfma_f32_16x16x4_f32, float32 throughout), one 16-channel
+            // block at a time.  K runs over the lanes in two halves (the
+            // weight buffer holds one half-wave's columns); within a half,
+            // K block kb of the 16x16x4 tile takes lanes {kb, 8 + kb,
+            // 16 + kb, 24 + kb}, so every lane reads its 8 operands of a
+            // tile as 2 float4.
             if (__ballot(in_r) != 0) {
-                const int g = lane >> 4, m = lane & 15;
+                // lane id re-read here so that the operand addresses below are
+                // not hoisted out of the step loop (they would stay live
+                // through the register-heavy sample evaluation)
+                int ln;
+                __asm__ volatile("v_mov_b32 %0, %1" : "=v"(ln) : "v"(lane));
+                const int g = ln >> 4, m = ln & 15;
+                const int k0 = (2 * g) ^ ((ln >> 1) & 7), k1 = k0 ^ 1;
                 const int nmb = (nv + 15) >> 4;
 #pragma unroll
                 for (int nb = 0; nb < CP / 16; ++nb) {
-                    const float4 *sb =
-                        reinterpret_cast<const float4 *>(gt + (16 * nb + m) * kGradStride + 16 * g);
-                    const float4 b0 = sb[0], b1 = sb[1], b2 = sb[2], b3 = sb[3];
+                    floatx4 acc[4];
+#pragma unroll
+                    for (int mb = 0; mb < 4; ++mb) acc[mb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                    // K in two halves: the weight columns of lanes 0-31, then 32-63
+                    for (int h = 0; h < 2; ++h) {
+                        const bool mine = in_r && (lane >> 5) == h;
+                        if (mine) {
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) wt[wofs[k]] = w[k];
+                        }
+                        wave_lds_order();
+                        const float4 *sb = reinterpret_cast<const float4 *>(
+                            gt + (16 * nb + m) * kGradStride + kHalf * h + 8 * g);
+                        const float4 b0 = sb[0], b1 = sb[1];
+#pragma unroll
+                        for (int mb = 0; mb < 4; ++mb) {
+                            if (mb < nmb) {
+                                const float4 *sa =
+                                    reinterpret_cast<const float4 *>(wt + (16 * mb + m) * kHalf);
+                                acc[mb] = mfma4(sa[k1], b1, mfma4(sa[k0], b0, acc[mb]));
+                            }
+                        }
+                        wave_lds_order();
+                        if (mine) {
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) wt[wofs[k]] = 0.0f;
+                        }
+                    }
                     const int ch = 16 * nb + m;
-                    for (int mb = 0; mb < nmb; ++mb) {
-                        const float4 *sa =
-                            reinterpret_cast<const float4 *>(wt + (16 * mb + m) * kWave);
-                        const float4 a0 = sa[k0], a1 = sa[k1], a2 = sa[k2], a3 = sa[k3];
-                        floatx4 d = {0.0f, 0.0f, 0.0f, 0.0f};
-                        d = mfma4(a0, b0, d);
-                        d = mfma4(a1, b1, d);
-                        d = mfma4(a2, b2, d);
-                        d = mfma4(a3, b3, d);
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            const int v = 16 * mb + 4 * g + i;
-                            const int vo = __shfl(vofs, v, kWave);
-                            if (v < nv && ch < C && d[i] != 0.0f)
-                                atomic_addf(gdata + (vbase + vo) * C + ch, d[i]);
+                    for (int mb = 0; mb < 4; ++mb) {
+                        if (mb < nmb) {
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                const int v = 16 * mb + 4 * g + i;
+                                const int vo = __shfl(vofs, v, kWave);
+                                const float d = acc[mb][i];
+                                if (v < nv && ch < C && d != 0.0f)
+                                    atomic_addf(gdata + (vbase + vo) * C + ch, d);
+                            }
                         }
                     }
                 }
             }
-            // flush the cell box line by line: (sz, sy) lines of cells are
-            // contiguous both in LDS and in the quadric grid
-            for (int sz = 0; sz + 1 < ez; ++sz) {
-                for (int sy = 0; sy + 1 < ey; ++sy) {
-                    const int sbase = sy * ex + sz * sxy;
-                    const int64_t vrow0 = vbase + (int64_t)sy * R + (int64_t)sz * RR;
-                    for (int i = lane; i < (ex - 1) * 7; i += kWave) {
-                        const int sx = i / 7, k = i - sx * 7;
-                        float *a = qacc + (sbase + sx) * 8 + k;
-                        const float v = *a;
-                        if (v != 0.0f) {
-                            *a = 0.0f;
-                            atomic_addf(gq + (vrow0 + sx) * 7 + k, v);
-                        }
-                    }
-                }
-            }
-            // restore the all-zero weight matrix for the next round / step
-            if (in_r) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) wt[wofs[k]] = 0.0f;
-            }
-            __syncthreads();
+            wave_lds_order();
             pend_r = pend_r && !in_r;
-            pend_s = pend_s && !in_s;
         }
     }
 }
 
 template <int NC>
 constexpr size_t tile_lds_bytes() {
-    return (size_t)(tile_channels<NC>() * kGradStride + kCapV * kWave + kCapV * 8 + kWave * 16) *
-           sizeof(float);
+    return (size_t)(tile_channels<NC>() * kGradStride + kCapV * kHalf) * sizeof(float);
 }
 
 static int64_t fused_waves(uint32_t W, uint32_t H, int n_cams, const int32_t *ray_pixel,
@@ -703,14 +746,18 @@ static int fused_forward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t 
     const float4 *rs = reinterpret_cast<const float4 *>(ray_state);
     int64_t waves = fused_waves(width, height, n_cams, ray_pixel, n_rays);
     unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
-    if (channels == 32)
-        fused_fwd_kernel<9><<<blocks, kFusedBlock, 0, s>>>(
-            cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
-            data_grid, reso, interval, logistic_coef, color, depth, rs, tlist, K, bins);
-    else
-        fused_fwd_kernel<16><<<blocks, kFusedBlock, 0, s>>>(
-            cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
-            data_grid, reso, interval, logistic_coef, color, depth, rs, tlist, K, bins);
+#define QSDVR_FWD(NC, BINS)                                                                  \
+    fused_fwd_kernel<NC, BINS><<<blocks, kFusedBlock, 0, s>>>(                               \
+        cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid, data_grid, \
+        reso, interval, logistic_coef, color, depth, rs, tlist, K, bins)
+    if (channels == 32) {
+        if (bins) QSDVR_FWD(9, true);
+        else QSDVR_FWD(9, false);
+    } else {
+        if (bins) QSDVR_FWD(16, true);
+        else QSDVR_FWD(16, false);
+    }
+#undef QSDVR_FWD
     QSDVR_LAUNCHED();
     return QSDVR_OK;
 }
