@@ -360,7 +360,8 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_kernel(
 // atomics on a cell box serialised on the same words.  Steps whose
 // appearance box exceeds kCapV vertices (about 4% of the
 // uniform steps at 800x800, 12% of proposal-guided steps, mostly silhouette
-// tiles) are split into rounds of 3x3x3-cell boxes (4x4x4 = 64 vertices).
+// tiles) are split into rounds of boxes that fit: slabs of the tight box
+// along its longest axis, or 3x3x3-cell boxes (4x4x4 = 64 vertices).
 constexpr int kCapV = 64;
 static_assert(kCapV >= 64, "a fallback round's 3x3x3-cell box has 64 vertices");
 
@@ -563,9 +564,9 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
         row_run_scatter<6>(gn7, act_r ? (int)rp.cell : -1, lane, gq);
         bool pend_r = act_r;  // appearance gradient not yet accumulated
         // Rounds: the tight box of all pending cells when it fits in kCapV
-        // vertices (the common case: one round); otherwise the 3x3x3-cell
-        // box centred on the first pending lane's cell.  Every round takes
-        // at least that lane's part, so the loop ends.
+        // vertices (the common case: one round); otherwise a part of it that
+        // fits (below).  Every round takes at least one pending lane, so the
+        // loop ends.
         for (;;) {
             int lx = INT_MAX, ly = INT_MAX, lz = INT_MAX, hx = INT_MIN, hy = INT_MIN, hz = INT_MIN;
             if (pend_r) {
@@ -575,15 +576,31 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
             lx = wave_min(lx); ly = wave_min(ly); lz = wave_min(lz);
             hx = wave_max(hx); hy = wave_max(hy); hz = wave_max(hz);
             if (hx < lx) break;  // nothing pending (wave-uniform)
-            if ((int64_t)(hx - lx + 2) * (hy - ly + 2) * (hz - lz + 2) > kCapV) {
-                const unsigned long long m = __ballot(pend_r);
-                const int leader = __ffsll((long long)m) - 1;
-                const int ax = __shfl(rp.ix, leader, kWave);
-                const int ay = __shfl(rp.iy, leader, kWave);
-                const int az = __shfl(rp.iz, leader, kWave);
-                lx = ax - 1; hx = ax + 1;
-                ly = ay - 1; hy = ay + 1;
-                lz = az - 1; hz = az + 1;
+            const int cx = hx - lx + 1, cy = hy - ly + 1, cz = hz - lz + 1;  // cells
+            if ((cx + 1) * (cy + 1) * (cz + 1) > kCapV) {
+                // Oversize: the slab of the tight box at its low end along the
+                // longest axis whose cross-section leaves room for two vertex
+                // layers, with as many cell layers as fit; else the 3x3x3-cell
+                // box around the first pending lane.  Either holds a pending lane.
+                const int sx = (cy + 1) * (cz + 1), sy = (cx + 1) * (cz + 1),
+                          sz = (cx + 1) * (cy + 1);
+                const bool fx = 2 * sx <= kCapV, fy = 2 * sy <= kCapV, fz = 2 * sz <= kCapV;
+                if (fz && cz >= (fx ? cx : 0) && cz >= (fy ? cy : 0)) {
+                    hz = lz + kCapV / sz - 2;
+                } else if (fy && cy >= (fx ? cx : 0)) {
+                    hy = ly + kCapV / sy - 2;
+                } else if (fx) {
+                    hx = lx + kCapV / sx - 2;
+                } else {
+                    const unsigned long long m = __ballot(pend_r);
+                    const int leader = __ffsll((long long)m) - 1;
+                    const int ax = __shfl(rp.ix, leader, kWave);
+                    const int ay = __shfl(rp.iy, leader, kWave);
+                    const int az = __shfl(rp.iz, leader, kWave);
+                    lx = ax - 1; hx = ax + 1;
+                    ly = ay - 1; hy = ay + 1;
+                    lz = az - 1; hz = az + 1;
+                }
             }
             const int ex = hx - lx + 2, ey = hy - ly + 2, ez = hz - lz + 2;
             const int sxy = ex * ey;

@@ -38,17 +38,12 @@ typedef short v8s __attribute__((ext_vector_type(8)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// float -> bfloat16 bits, round to nearest even
+// float -> bfloat16 bits, round to nearest even (v_cvt_pk_bf16_f32)
 __device__ __forceinline__ unsigned short bf16_bits(float f) {
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);  // inf / nan
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
+    return __builtin_bit_cast(unsigned short, (__bf16)f);
 }
 
-__device__ __forceinline__ float bf16_round(float f) {
-    return __uint_as_float((unsigned)bf16_bits(f) << 16);
-}
+__device__ __forceinline__ float bf16_round(float f) { return (float)(__bf16)f; }
 
 // canonical PE index of chain slot s (0..15) of axis a, or -1 (zero pad):
 // slot 0 = coordinate, 1 + 2k = sin(2^k pi c), 2 + 2k = cos(2^k pi c)
@@ -94,10 +89,10 @@ __global__ void __launch_bounds__(256) pack_kernel(const float *__restrict__ the
 }
 
 __device__ __forceinline__ v8s pack8(const float v[8]) {
-    v8s r;
+    v8bf r;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) r[i] = (short)bf16_bits(v[i]);
-    return r;
+    for (int i = 0; i < 8; ++i) r[i] = (__bf16)v[i];
+    return __builtin_bit_cast(v8s, r);
 }
 
 __device__ __forceinline__ v4f mfma(v8s a, v8s b, v4f c) {
@@ -105,17 +100,48 @@ __device__ __forceinline__ v4f mfma(v8s a, v8s b, v4f c) {
                                                    __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
 }
 
-// One hidden layer on a 16-point tile: D[mt] = W^T[mt, :] x B + bias.
-__device__ __forceinline__ void layer(const v8s *__restrict__ A /*[4][2][64]*/, const v8s B[2],
-                                      const float *__restrict__ bias, int lane, v4f D[4]) {
+// One hidden layer on two 16-point tiles: D[t][mt] = W^T[mt, :] x B[t] + bias.
+// Each weight fragment is read from LDS once for both tiles, and the two
+// tiles' MFMA chains are independent.
+__device__ __forceinline__ void layer2(const v8s *__restrict__ A /*[4][2][64]*/, const v8s B[2][2],
+                                       const float *__restrict__ bias, int lane, v4f D[2][4], int zofs) {
     const int g = lane >> 4;
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-        v4f acc = {bias[16 * mt + 4 * g + 0], bias[16 * mt + 4 * g + 1],
-                   bias[16 * mt + 4 * g + 2], bias[16 * mt + 4 * g + 3]};
+        const v4f b = {bias[16 * mt + 4 * g + 0], bias[16 * mt + 4 * g + 1],
+                       bias[16 * mt + 4 * g + 2], bias[16 * mt + 4 * g + 3]};
+        v4f acc0 = b, acc1 = b;
 #pragma unroll
-        for (int kb = 0; kb < 2; ++kb) acc = mfma(A[(mt * 2 + kb) * 64 + lane], B[kb], acc);
-        D[mt] = acc;
+        for (int kb = 0; kb < 2; ++kb) {
+            const v8s w = A[(mt * 2 + kb) * 64 + lane + zofs];
+            acc0 = mfma(w, B[0][kb], acc0);
+            acc1 = mfma(w, B[1][kb], acc1);
+        }
+        D[0][mt] = acc0;
+        D[1][mt] = acc1;
+    }
+}
+
+// Positional-encoding chain of axis g at the point ori + dir t (lane group 3
+// carries zeros): [p, sin(pi p), cos(pi p), ..., sin(32 pi p), cos(32 pi p), 0, 0, 0]
+__device__ __forceinline__ void pe_chain(float4 sa, float4 sb, float t, int g, float chain[16]) {
+    const float c = g == 0 ? sa.x + sb.x * t
+                           : (g == 1 ? sa.y + sb.y * t : (g == 2 ? sa.z + sb.z * t : 0.0f));
+    float sn, cs;
+    sincospif(c, &sn, &cs);
+    chain[0] = c;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        chain[1 + 2 * k] = sn;
+        chain[2 + 2 * k] = cs;
+        const float s2 = 2.0f * sn * cs, c2 = 1.0f - 2.0f * sn * sn;
+        sn = s2;
+        cs = c2;
+    }
+    chain[13] = chain[14] = chain[15] = 0.0f;
+    if (g == 3) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) chain[i] = 0.0f;
     }
 }
 
@@ -161,78 +187,80 @@ __global__ void __launch_bounds__(kMlpBlock) mlp_fwd_kernel(
     for (int64_t r = wave0; r < n_rays; r += nwaves) {
         const float4 sa = ray_state[2 * r], sb = ray_state[2 * r + 1];
         const uint32_t count = __float_as_uint(sa.w);
+        if (!train && count < 2) {  // no segment: logits unused, zero them
+            logits[r * kBins + lane] = 0.0f;
+            continue;
+        }
         const float L = count >= 2 ? (float)(count - 1) * interval : 0.0f;
 #pragma unroll 1
-        for (int tb = 0; tb < kBins / 16; ++tb) {
-            const int bin = 16 * tb + j;
-            const float t = ((float)bin + 0.5f) * (1.0f / kBins) * L;
-            // positional-encoding chain of axis g (lane group 3 carries zeros)
-            float chain[16];
-            {
-                const float c = g == 0 ? sa.x + sb.x * t
-                                       : (g == 1 ? sa.y + sb.y * t : (g == 2 ? sa.z + sb.z * t : 0.0f));
-                float sn, cs;
-                sincospif(c, &sn, &cs);
-                chain[0] = c;
+        for (int tp = 0; tp < kBins / 32; ++tp) {
+            // two 16-point tiles: bins 32 tp + j and 32 tp + 16 + j
+            v8s B[2][2];
+            int64_t pt[2];
 #pragma unroll
-                for (int k = 0; k < 6; ++k) {
-                    chain[1 + 2 * k] = sn;
-                    chain[2 + 2 * k] = cs;
-                    const float s2 = 2.0f * sn * cs, c2 = 1.0f - 2.0f * sn * sn;
-                    sn = s2;
-                    cs = c2;
-                }
-                chain[13] = chain[14] = chain[15] = 0.0f;
-                if (g == 3) {
+            for (int u = 0; u < 2; ++u) {
+                const int bin = 32 * tp + 16 * u + j;
+                const float t = ((float)bin + 0.5f) * (1.0f / kBins) * L;
+                float chain[16];
+                pe_chain(sa, sb, t, g, chain);
+                B[u][0] = pack8(chain);
+                B[u][1] = pack8(chain + 8);
+                pt[u] = r * kBins + bin;
+                if (train) {
 #pragma unroll
-                    for (int i = 0; i < 16; ++i) chain[i] = 0.0f;
+                    for (int s = 0; s < 13; ++s) {
+                        const int f = pe_index(g, s);
+                        if (f >= 0) x0s[pt[u] * 64 + f] = bf16_round(chain[s]);
+                    }
                 }
             }
-            v8s B[2];
-            B[0] = pack8(chain);
-            B[1] = pack8(chain + 8);
-            const int64_t pt = r * kBins + bin;
-            if (train) {
-#pragma unroll
-                for (int s = 0; s < 13; ++s) {
-                    const int f = pe_index(g, s);
-                    if (f >= 0) x0s[pt * 64 + f] = bf16_round(chain[s]);
-                }
-            }
-            v4f D[4];
+            v4f D[2][4];
             float act[16];
-            layer(sA, B, sT, lane, D);
-            to_operand(D, B, act);
-            if (train) {
+            // opaque zero offset: keeps the weight-fragment reads inside the
+            // loop rather than hoisted into ~100 registers for the kernel
+            int zofs;
+            __asm__ volatile("v_mov_b32 %0, 0" : "=v"(zofs));
+            layer2(sA, B, sT, lane, D, zofs);
 #pragma unroll
-                for (int kb = 0; kb < 2; ++kb)
+            for (int u = 0; u < 2; ++u) {
+                to_operand(D[u], B[u], act);
+                if (train) {
 #pragma unroll
-                    for (int s = 0; s < 8; ++s)
-                        x1s[pt * 64 + hidden_index(kb, g, s)] = act[8 * kb + s];
-            }
-            layer(sA + 4 * 2 * 64, B, sT + 64, lane, D);
-            to_operand(D, B, act);
-            if (train) {
+                    for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-                for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-                    for (int s = 0; s < 8; ++s)
-                        x2s[pt * 64 + hidden_index(kb, g, s)] = act[8 * kb + s];
-            }
-            layer(sA + 2 * 4 * 2 * 64, B, sT + 128, lane, D);
-            float z = 0.0f;
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int f = 16 * mt + 4 * g + q;
-                    const float h = fmaxf(D[mt][q], 0.0f);
-                    if (train) h3s[pt * 64 + f] = h;
-                    z += sT[192 + f] * h;
+                        for (int s = 0; s < 8; ++s)
+                            x1s[pt[u] * 64 + hidden_index(kb, g, s)] = act[8 * kb + s];
                 }
-            z += __shfl_xor(z, 16, kWave);
-            z += __shfl_xor(z, 32, kWave);
-            if (g == 0) logits[pt] = z + sT[256];
+            }
+            layer2(sA + 4 * 2 * 64, B, sT + 64, lane, D, zofs);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                to_operand(D[u], B[u], act);
+                if (train) {
+#pragma unroll
+                    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+                        for (int s = 0; s < 8; ++s)
+                            x2s[pt[u] * 64 + hidden_index(kb, g, s)] = act[8 * kb + s];
+                }
+            }
+            layer2(sA + 2 * 4 * 2 * 64, B, sT + 128, lane, D, zofs);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                float z = 0.0f;
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int f = 16 * mt + 4 * g + q;
+                        const float h = fmaxf(D[u][mt][q], 0.0f);
+                        if (train) h3s[pt[u] * 64 + f] = h;
+                        z += sT[192 + f] * h;
+                    }
+                z += __shfl_xor(z, 16, kWave);
+                z += __shfl_xor(z, 32, kWave);
+                if (g == 0) logits[pt[u]] = z + sT[256];
+            }
         }
     }
 }
@@ -413,7 +441,12 @@ __global__ void __launch_bounds__(256) sampler_kernel(const float *__restrict__ 
     const int64_t r = (int64_t)blockIdx.x * 4 + wv;
     if (r >= n_rays) return;  // whole wavefront
     const uint32_t count = __float_as_uint(ray_state[2 * r].w);
-    const float L = count >= 2 ? (float)(count - 1) * interval : 0.0f;
+    float *out = tlist + r * K;
+    if (count < 2) {  // no segment: all samples at the origin
+        for (int k = lane; k < K; k += kWave) out[k] = 0.0f;
+        return;
+    }
+    const float L = (float)(count - 1) * interval;
     const float z = logits[r * kBins + lane];
     const float d = z > 15.0f ? z : log1pf(expf(z));
     const float S = wave_scan_incl(d, lane);
@@ -427,7 +460,6 @@ __global__ void __launch_bounds__(256) sampler_kernel(const float *__restrict__ 
     s_pdf[wv][lane] = pdf;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    float *out = tlist + r * K;
     const int nm = K - 2;
     for (int k = lane; k < nm; k += kWave) {
         const float u = ((float)k + 0.5f) / (float)nm;
