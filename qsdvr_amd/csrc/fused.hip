@@ -404,8 +404,9 @@ __device__ __forceinline__ void run_scan_step(float v[7], bool &f, int x) {
 }
 
 // Adds the first N of v[] to quadric cell `key` (key < 0: nothing), summed
-// first over runs of equal keys along each 8-lane pixel row; the run's last
-// lane issues the global atomics.  Must be called by all 64 lanes.
+// first over runs of equal keys along each 8-lane pixel row and then over
+// such runs in different rows; one lane per merged group issues the global
+// atomics.  Must be called by all 64 lanes.
 template <int N>
 __device__ __forceinline__ void row_run_scatter(float v[7], int key, int lane,
                                                 float *__restrict__ gq) {
@@ -417,7 +418,26 @@ __device__ __forceinline__ void row_run_scatter(float v[7], int key, int lane,
     run_scan_step<N, 1>(v, f, x);
     run_scan_step<N, 2>(v, f, x);
     run_scan_step<N, 4>(v, f, x);
-    if (key >= 0 && (x == 7 || next_head)) {
+    // the last lane of each run holds its sum; merge the runs of equal keys
+    // in different rows (butterfly over the row bits: of two partners with
+    // the same key the lower lane takes the sum, the upper drops out)
+    int tail = key >= 0 && (x == 7 || next_head) ? key : -1;
+#pragma unroll
+    for (int m = 8; m < kWave; m <<= 1) {
+        const int pk = __shfl_xor(tail, m, kWave);
+        float p[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) p[k] = __shfl_xor(v[k], m, kWave);
+        if (tail >= 0 && pk == tail) {
+            if (lane & m) {
+                tail = -1;
+            } else {
+#pragma unroll
+                for (int k = 0; k < N; ++k) v[k] += p[k];
+            }
+        }
+    }
+    if (tail >= 0) {
         float *dst = gq + 7 * (int64_t)key;
 #pragma unroll
         for (int k = 0; k < N; ++k)
