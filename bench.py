@@ -71,7 +71,7 @@ def main():
     import numpy as np
     import torch
 
-    from qsdvr_amd import fused, scenes
+    from qsdvr_amd import fused, proposal, scenes
     from qsdvr_amd import render as hr
     from qsdvr_amd import qsdvr
     from qsdvr_amd.camera import Camera
@@ -180,6 +180,25 @@ def main():
                 "traffic": None, "algorithmic_bytes_per_launch": byts,
                 "avg_launch_ms": round(kern_ms[dom], 4)}
 
+    # ------------------------------------- op-surface byte equivalent ----
+    # The op-surface form of the same step moves (344 + 20 C) bytes per render
+    # sample plus 150 B per ray plus the grids and gradient grids (4 G / P per
+    # ray) through HBM; at the fused path's rays/s that is the bandwidth the
+    # op-surface form would need.
+    renders = 0
+    for i in range(args.steps):
+        p = (args.warmup + i) % w.n_poses
+        rs = proposal.ray_state(cam_buf[p * CAM_BYTES:(p + 1) * CAM_BYTES], 1, w.width, w.height,
+                                w.interval)
+        cnt = proposal.ray_counts(rs).to(torch.int64)
+        renders += int(torch.clamp(cnt - 1, min=0).sum())
+    s_r = renders / (args.steps * P)
+    G_bytes = 28 * w.R ** 3 + 4 * 32 * (w.R + 1) ** 3
+    b_op = (344 + 20 * 32) * s_r + 150 + 4 * G_bytes / P
+    op_equiv = {"bytes_per_ray": round(b_op, 1), "render_samples_per_ray": round(s_r, 2),
+                "effective_GBs": round(b_op * rays_s / 1e9, 1),
+                "frac_of_peak": round(b_op * rays_s / 1e9 / HBM_PEAK_GBS, 4)}
+
     # --------------------------------------------------- CPU baseline ----
     cpu = None
     if args.cpu_rays > 0:
@@ -197,7 +216,7 @@ def main():
            "op_surface_rays_per_s": round(op_rays, 1) if op_rays else None,
            "proposal": prop_results,
            "kernel_ms": {k: round(v, 4) for k, v in kern_ms.items()},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "op_surface_equivalent": op_equiv, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
