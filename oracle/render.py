@@ -19,8 +19,11 @@ Semantic contract (follow render_torch.py; fix only what stops it running):
 Pinned by (tests/test_oracle_render.py): the closed-form sphere grid from
 ``init_as_sphere`` (render_torch.py:22-28), plane special cases for SDF and
 normal, constant / indicator grids for interpolation and the transpose
-(dot-product) test for its adjoint, real-SH closed forms (scipy) and the
-metallic=0 / coef0-only shading cases, single-step SDF compositing and the
+(dot-product) test for its adjoint, real-SH closed forms and the
+metallic=0 / coef0-only shading cases, for degree 3 (53 channels, no
+reference counterpart) orthonormality on the sphere, harmonicity and the
+textbook z-pole table under the axis substitution that maps it onto the
+reference's degree-2 table, single-step SDF compositing and the
 invariants T non-increasing / sum of weights <= 1, a literal per-ray Python
 loop for compositing, and float64 central finite differences for every
 gradient.
@@ -37,6 +40,15 @@ import torch
 SH_FACTOR_F32 = np.array([0.2820947917, 0.4886025119, 0.4886025119, 0.4886025119,
                           1.0925484305, 1.0925484305, 0.3153915652, 1.0925484305,
                           0.5462742152], dtype=np.float32)
+
+# Degree 3 (no reference counterpart): the unsigned real SH of degree 3 in
+# the same y-pole family and m = -3..3 order as the reference's degree <= 2
+# table, normalisations sqrt(35/32pi), sqrt(105/4pi), sqrt(21/32pi),
+# sqrt(7/16pi), sqrt(21/32pi), sqrt(105/16pi), sqrt(35/32pi) as float32 of
+# their 10-digit decimals.
+SH3_FACTOR_F32 = np.array([0.5900435899, 2.8906114426, 0.4570457995, 0.3731763326,
+                           0.4570457995, 1.4453057213, 0.5900435899], dtype=np.float32)
+SH_FACTOR16_F32 = np.concatenate([SH_FACTOR_F32, SH3_FACTOR_F32])
 
 
 def layers_to_grid(x, y, z, off):
@@ -129,30 +141,45 @@ def grid_interpolation(data, pts, idx, R):
     return _lerp(a0, a1, pts[:, 2])
 
 
-def sh_basis(v):
-    """The nine degree-2 basis terms of render_torch.py:186-195, without the
-    constants: [1, x, y, z, xz, xy, 2y^2 - z^2 - x^2, yz, z^2 - x^2]."""
+def sh_basis(v, nc=9):
+    """The basis polynomials without constants.  nc = 9: the degree-2 terms
+    of render_torch.py:186-195, [1, x, y, z, xz, xy, 2y^2 - z^2 - x^2, yz,
+    z^2 - x^2]; nc = 16 appends degree 3, [x(3z^2 - x^2), xyz,
+    x(4y^2 - x^2 - z^2), y(2y^2 - 3x^2 - 3z^2), z(4y^2 - x^2 - z^2),
+    y(z^2 - x^2), z(z^2 - 3x^2)]."""
     x, y, z = v[:, 0], v[:, 1], v[:, 2]
     zz, xx = z * z, x * x
-    return torch.stack([torch.ones_like(x), x, y, z, x * z, x * y,
-                        2.0 * y * y - zz - xx, y * z, zz - xx], dim=1)
+    terms = [torch.ones_like(x), x, y, z, x * z, x * y, 2.0 * y * y - zz - xx, y * z, zz - xx]
+    if nc == 16:
+        yy = y * y
+        terms += [x * (3.0 * zz - xx), x * y * z, x * (4.0 * yy - xx - zz),
+                  y * (2.0 * yy - 3.0 * xx - 3.0 * zz), z * (4.0 * yy - xx - zz), y * (zz - xx),
+                  z * (zz - 3.0 * xx)]
+    elif nc != 9:
+        raise ValueError("nc must be 9 or 16")
+    return torch.stack(terms, dim=1)
 
 
 def sh_irradiance(v, coef, factors=None):
-    """render_torch.py:174-195.  v [S,3], coef [S,9,3] -> [S,3]."""
-    K = torch.as_tensor(SH_FACTOR_F32 if factors is None else factors, dtype=v.dtype)
-    Y = sh_basis(v) * K
+    """render_torch.py:174-195.  v [S,3], coef [S,NC,3] -> [S,3] (NC = 9, or
+    16 for degree 3)."""
+    nc = coef.shape[1]
+    default = SH_FACTOR_F32 if nc == 9 else SH_FACTOR16_F32
+    K = torch.as_tensor(default if factors is None else factors, dtype=v.dtype)
+    Y = sh_basis(v, nc) * K
     return (coef * Y.unsqueeze(-1)).sum(dim=1)
 
 
 def shade(normal, view, data):
     """render_torch.py:144-152.  Returns (rgb [S,3], specularL [S,3]).
-    Channels: [3k+c] SH coef k colour c (k<9), [27:30] diffuse,
-    [30] metallic, [31] ao."""
-    ao = data[:, 31:32]
-    diffuse = data[:, 27:30]
-    metallic = data[:, 30:31]
-    specular = data[:, :27].reshape(-1, 9, 3)
+    Channels (C = 3 NC + 5): [3k+c] SH coef k colour c (k < NC), then
+    diffuse RGB, metallic, ao; C = 32 (NC = 9) as in the reference, or 53
+    (NC = 16, degree 3)."""
+    nc = (data.shape[1] - 5) // 3
+    ao = data[:, 3 * nc + 4:3 * nc + 5]
+    diffuse = data[:, 3 * nc:3 * nc + 3]
+    metallic = data[:, 3 * nc + 3:3 * nc + 4]
+    specular = data[:, :3 * nc].reshape(-1, nc, 3)
     vdotn = (view * normal).sum(dim=1).reshape(-1, 1)
     reflect = view - normal * (2.0 * vdotn)
     L = sh_irradiance(reflect, specular)

@@ -12,8 +12,14 @@
 namespace qsdvr {
 
 // Basis terms Y_k(v) (with their constants) for k < NC.
-// k < 9 : models/render_torch.py:174-195 (y-pole real SH, degree <= 2)
-// k >= 9: degree-3 real SH of the same family (y as the pole).
+// k < 9 : models/render_torch.py:174-195.  That table is the unsigned real SH
+//         of degree <= 2 in the order m = -l..l, with the pole on y: the
+//         z-pole polynomials under (x, y, z) -> (z, x, y).
+// k >= 9: degree 3 of the same family, m = -3..3:
+//         x(3z^2 - x^2), xyz, x(4y^2 - x^2 - z^2), y(2y^2 - 3x^2 - 3z^2),
+//         z(4y^2 - x^2 - z^2), y(z^2 - x^2), z(z^2 - 3x^2), with the
+//         normalisations sqrt(35/32pi), sqrt(105/4pi), sqrt(21/32pi),
+//         sqrt(7/16pi), sqrt(21/32pi), sqrt(105/16pi), sqrt(35/32pi).
 template <int NC>
 __device__ __forceinline__ void sh_basis(f3 v, float Y[NC]) {
     const float x = v.x, y = v.y, z = v.z;
@@ -29,13 +35,13 @@ __device__ __forceinline__ void sh_basis(f3 v, float Y[NC]) {
     Y[8] = QSDVR_SH_K8 * (zz - xx);
     if constexpr (NC > 9) {
         const float yy = y * y;
-        Y[9] = QSDVR_SH_K9 * (z * (3.0f * xx - zz));
+        Y[9] = QSDVR_SH_K9 * (x * (3.0f * zz - xx));
         Y[10] = QSDVR_SH_K10 * (x * y * z);
-        Y[11] = QSDVR_SH_K11 * (z * (4.0f * yy - xx - zz));
+        Y[11] = QSDVR_SH_K11 * (x * (4.0f * yy - xx - zz));
         Y[12] = QSDVR_SH_K12 * (y * (2.0f * yy - 3.0f * xx - 3.0f * zz));
-        Y[13] = QSDVR_SH_K11 * (x * (4.0f * yy - xx - zz));
-        Y[14] = QSDVR_SH_K14 * (y * (xx - zz));
-        Y[15] = QSDVR_SH_K9 * (x * (xx - 3.0f * zz));
+        Y[13] = QSDVR_SH_K11 * (z * (4.0f * yy - xx - zz));
+        Y[14] = QSDVR_SH_K14 * (y * (zz - xx));
+        Y[15] = QSDVR_SH_K9 * (z * (zz - 3.0f * xx));
     }
 }
 
@@ -50,15 +56,15 @@ __device__ __forceinline__ f3 sh_basis_vjp(f3 v, const float gc[NC]) {
     float gz = gc[3] + x * gc[4] + y * gc[7] + 2.0f * z * (gc[8] - gc[6]);
     if constexpr (NC > 9) {
         const float xx = x * x, yy = y * y, zz = z * z;
-        gx += gc[9] * (6.0f * x * z) + gc[10] * (y * z) + gc[11] * (-2.0f * x * z) +
-              gc[12] * (-6.0f * x * y) + gc[13] * (4.0f * yy - 3.0f * xx - zz) +
-              gc[14] * (2.0f * x * y) + gc[15] * (3.0f * xx - 3.0f * zz);
-        gy += gc[10] * (x * z) + gc[11] * (8.0f * y * z) +
-              gc[12] * (6.0f * yy - 3.0f * xx - 3.0f * zz) + gc[13] * (8.0f * x * y) +
-              gc[14] * (xx - zz);
-        gz += gc[9] * (3.0f * xx - 3.0f * zz) + gc[10] * (x * y) +
-              gc[11] * (4.0f * yy - xx - 3.0f * zz) + gc[12] * (-6.0f * y * z) +
-              gc[13] * (-2.0f * x * z) + gc[14] * (-2.0f * y * z) + gc[15] * (-6.0f * x * z);
+        gx += gc[9] * (3.0f * zz - 3.0f * xx) + gc[10] * (y * z) +
+              gc[11] * (4.0f * yy - 3.0f * xx - zz) + gc[12] * (-6.0f * x * y) +
+              gc[13] * (-2.0f * x * z) + gc[14] * (-2.0f * x * y) + gc[15] * (-6.0f * x * z);
+        gy += gc[10] * (x * z) + gc[11] * (8.0f * x * y) +
+              gc[12] * (6.0f * yy - 3.0f * xx - 3.0f * zz) + gc[13] * (8.0f * y * z) +
+              gc[14] * (zz - xx);
+        gz += gc[9] * (6.0f * x * z) + gc[10] * (x * y) + gc[11] * (-2.0f * x * z) +
+              gc[12] * (-6.0f * y * z) + gc[13] * (4.0f * yy - xx - 3.0f * zz) +
+              gc[14] * (2.0f * y * z) + gc[15] * (3.0f * zz - 3.0f * xx);
     }
     return f3{gx, gy, gz};
 }

@@ -139,6 +139,81 @@ def test_sh_matches_independent_table():
     assert [float(np.float32(x)) for x in exact] == [float(orc.SH_FACTOR_F32[i]) for i in (0, 1, 4, 6, 8)]
 
 
+def _sphere_quadrature(n=12, m=24):
+    """Gauss-Legendre in y (the pole) times uniform azimuth: exact for the
+    products of two degree-3 polynomials on the unit sphere."""
+    ys, wy = np.polynomial.legendre.leggauss(n)
+    ph = np.arange(m) * 2 * np.pi / m
+    yy, pp = np.meshgrid(ys, ph, indexing="ij")
+    st = np.sqrt(1 - yy ** 2)
+    v = np.stack([(st * np.cos(pp)).ravel(), yy.ravel(), (st * np.sin(pp)).ravel()], 1)
+    w = np.outer(wy, np.full(m, 2 * np.pi / m)).ravel()
+    return torch.as_tensor(v, dtype=D), torch.as_tensor(w, dtype=D)
+
+
+def test_sh_degree3_orthonormal_harmonic_and_textbook():
+    """Degree 3 (no reference counterpart) is pinned three ways: the 16
+    functions with the float32 constants are orthonormal on the sphere; every
+    degree-3 polynomial is harmonic; and each equals the textbook z-pole
+    real SH of the same m (Sloan, "Stupid SH tricks") under the axis
+    substitution that maps the textbook's degree-2 table onto
+    render_torch's: P(x, y, z) = Q(z, x, y)."""
+    v, w = _sphere_quadrature()
+    K = torch.as_tensor(orc.SH_FACTOR16_F32, dtype=D)
+    Y = orc.sh_basis(v, 16) * K
+    G = Y.t() @ (Y * w[:, None])
+    assert torch.allclose(G, torch.eye(16, dtype=D), atol=1e-6)
+    exact = [math.sqrt(35 / (32 * math.pi)), math.sqrt(105 / (4 * math.pi)),
+             math.sqrt(21 / (32 * math.pi)), math.sqrt(7 / (16 * math.pi)),
+             math.sqrt(21 / (32 * math.pi)), math.sqrt(105 / (16 * math.pi)),
+             math.sqrt(35 / (32 * math.pi))]
+    assert [float(np.float32(x)) for x in exact] == [float(x) for x in orc.SH3_FACTOR_F32]
+    Ye = orc.sh_basis(v, 16)[:, 9:] * torch.as_tensor(exact, dtype=D)
+    Ge = Ye.t() @ (Ye * w[:, None])
+    assert torch.allclose(Ge, torch.eye(7, dtype=D), atol=1e-12)
+    # harmonic: the Laplacian of each homogeneous cubic vanishes
+    g = torch.Generator().manual_seed(12)
+    p = torch.randn((7, 3), generator=g, dtype=D).requires_grad_(True)
+    for k in range(9, 16):
+        f = lambda q: orc.sh_basis(q.reshape(1, 3), 16)[0, k]
+        for i in range(7):
+            H = torch.autograd.functional.hessian(f, p[i].detach())
+            assert abs(float(torch.trace(H))) < 1e-12, k
+    # textbook z-pole degree-3 table, m = -3..3, evaluated at (z, x, y)
+    x, y, z = v[:, 2], v[:, 0], v[:, 1]
+    Q = torch.stack([y * (3 * x * x - y * y), x * y * z, y * (4 * z * z - x * x - y * y),
+                     z * (2 * z * z - 3 * x * x - 3 * y * y), x * (4 * z * z - x * x - y * y),
+                     z * (x * x - y * y), x * (x * x - 3 * y * y)], 1)
+    assert torch.allclose(orc.sh_basis(v, 16)[:, 9:], Q, atol=1e-13)
+    # the same substitution maps the textbook degree <= 2 table onto render_torch's
+    Q2 = torch.stack([torch.ones_like(x), y, z, x, x * y, y * z, 2 * z * z - x * x - y * y,
+                      x * z, x * x - y * y], 1)
+    assert torch.allclose(orc.sh_basis(v, 9), Q2, atol=1e-13)
+
+
+def test_shade_degree3_extends_degree2():
+    """The 53-channel shading with all degree-3 coefficients zero equals the
+    reference's 32-channel shading of the same other channels."""
+    g = torch.Generator().manual_seed(13)
+    n = 25
+    nrm = torch.nn.functional.normalize(torch.randn((n, 3), generator=g, dtype=D), dim=1)
+    view = torch.nn.functional.normalize(torch.randn((n, 3), generator=g, dtype=D), dim=1)
+    d32 = torch.randn((n, 32), generator=g, dtype=D)
+    d53 = torch.zeros((n, 53), dtype=D)
+    d53[:, :27] = d32[:, :27]
+    d53[:, 48:53] = d32[:, 27:32]
+    rgb32, L32 = orc.shade(nrm, view, d32)
+    rgb53, L53 = orc.shade(nrm, view, d53)
+    assert torch.allclose(rgb32, rgb53, atol=1e-14) and torch.allclose(L32, L53, atol=1e-14)
+    # and a single degree-3 coefficient adds exactly its basis term
+    d53[:, 3 * 12 + 1] = 1.0  # coefficient 12, green
+    r = view - 2 * (view * nrm).sum(1, keepdim=True) * nrm
+    _, L = orc.shade(nrm, view, d53)
+    y = r[:, 1]
+    term = float(orc.SH3_FACTOR_F32[3]) * y * (2 * y * y - 3 * r[:, 0] ** 2 - 3 * r[:, 2] ** 2)
+    assert torch.allclose(L[:, 1] - L32[:, 1], term, atol=1e-12)
+
+
 def test_shading_special_cases():
     g = torch.Generator().manual_seed(4)
     n = 20
@@ -217,7 +292,7 @@ def test_clamp_gradient_passes_at_zero():
     assert sdf.grad[1].abs() > 0 and sdf.grad[0].abs() > 0
 
 
-@pytest.mark.parametrize("fn", ["sdf", "normal", "interp", "shade", "composite"])
+@pytest.mark.parametrize("fn", ["sdf", "normal", "interp", "shade", "shade53", "composite"])
 def test_finite_differences(fn):
     g = torch.Generator().manual_seed(6)
     R = 3
@@ -234,10 +309,10 @@ def test_finite_differences(fn):
         pts = torch.rand((6, 3), generator=g, dtype=D) * 2 - 1
         idx = torch.randint(0, R ** 3, (6,), generator=g)
         assert torch.autograd.gradcheck(lambda d: orc.grid_interpolation(d, pts, idx, R), (data,))
-    elif fn == "shade":
+    elif fn in ("shade", "shade53"):
         nrm = torch.nn.functional.normalize(torch.randn((5, 3), generator=g, dtype=D), dim=1)
         view = torch.nn.functional.normalize(torch.randn((5, 3), generator=g, dtype=D), dim=1)
-        data = torch.randn((5, 32), generator=g, dtype=D)
+        data = torch.randn((5, 32 if fn == "shade" else 53), generator=g, dtype=D)
         assert torch.autograd.gradcheck(lambda n, d: orc.shade(n, view, d)[0],
                                         (nrm.requires_grad_(True), data.requires_grad_(True)))
     else:
