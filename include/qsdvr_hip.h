@@ -299,6 +299,36 @@ int qsdvr_fused_backward_listed(const qsdvr_camera *cams, int32_t n_cams,
                                 float *data_grid_grad, int32_t scatter_mode,
                                 qsdvr_stream stream);
 
+/* Deterministic adjoint (sorted-segment gradient scatter).  No reference
+ * counterpart: the reference's backward kernels add every sample's gradient
+ * with float atomics (kernel/render.cu:121-152, kernel/surface.cu:74-146),
+ * whose order, and so whose rounding, varies run to run.  Same rays,
+ * samples and result as qsdvr_fused_backward_listed (tlist == NULL: the
+ * uniform samples), but the sums into sdf_grid_grad and data_grid_grad are
+ * bitwise reproducible: per-sample gradient records are radix-sorted by
+ * cell and every cell's run is summed in a fixed order (appearance runs in
+ * 8 cell-parity passes without atomics).  ray_state comes from
+ * qsdvr_ray_state.  sample_capacity bounds the total sdf samples of all
+ * rays (sum over rays of count, or of K when listed, for rays with count
+ * >= 2); the workspace holds qsdvr_fused_det_workspace_size() bytes for it.
+ * reso <= 512.  Synchronises the host once (reads back the sample total);
+ * QSDVR_EWORKSPACE if the total exceeds sample_capacity. */
+size_t qsdvr_fused_det_workspace_size(int64_t n_rays, int64_t sample_capacity,
+                                      int32_t channels, int32_t reso);
+int qsdvr_fused_backward_det(const qsdvr_camera *cams, int32_t n_cams,
+                             int32_t width, int32_t height,
+                             const int32_t *ray_pixel, int64_t n_rays,
+                             const float *ray_state, const float *tlist,
+                             int32_t K, const float *sdf_grid,
+                             const float *data_grid, int32_t reso,
+                             int32_t channels, float interval,
+                             float logistic_coef, const float *color,
+                             const float *depth, const float *grad_color,
+                             const float *grad_depth, float *sdf_grid_grad,
+                             float *data_grid_grad, int64_t sample_capacity,
+                             void *workspace, size_t workspace_bytes,
+                             qsdvr_stream stream);
+
 /* ---------------------------------------------------------- proposal --- */
 /* No reference counterpart.  Per-point MLP PE(39) -> 64 -> 64 -> 64 -> 1
  * (ReLU), bfloat16 weights on MFMA with float32 accumulation, evaluated at

@@ -69,6 +69,9 @@ SIGNATURES = {
                                     P, P, P], C.c_int),
     "qsdvr_fused_backward_listed": ([P, I32, I32, I32, P, I64, P, P, I32, P, P, I32, I32, F, F,
                                      P, P, P, P, P, P, I32, P], C.c_int),
+    "qsdvr_fused_det_workspace_size": ([I64, I64, I32, I32], SZ),
+    "qsdvr_fused_backward_det": ([P, I32, I32, I32, P, I64, P, P, I32, P, P, I32, I32, F, F,
+                                  P, P, P, P, P, P, I64, P, SZ, P], C.c_int),
     "qsdvr_proposal_param_count": ([], C.c_int32),
     "qsdvr_proposal_packed_bytes": ([], SZ),
     "qsdvr_proposal_pack": ([P, P, P], C.c_int),

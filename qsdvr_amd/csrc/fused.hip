@@ -15,6 +15,7 @@
 // sum_{i<=k} w_i rgb_i; the suffix the alpha adjoint needs is
 // saved_colour - prefix, and 1/(1 - alpha_k) = (1 + e_{k+1}) / (1 + e_k)
 // exactly (no division by a vanishing 1 - alpha).
+#include "det_scatter.h"
 #include "quadric.h"
 #include "raygen.h"
 #include "shade.h"
@@ -336,6 +337,128 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_kernel(
         float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
         quadric_sdf_grad(load_quadric(qgrid, cp0.cell), cp0.local, carry, inv_reso, gq7);
         scatter_quadric(gq, cp0.cell, gq7, 7);
+    }
+}
+
+// ------------------------------------------ deterministic-scatter adjoint ----
+// The same recompute adjoint, lane = ray, writing gradient records (keyed by
+// cell) instead of adding into the grids; det_scatter.hip sorts and sums
+// them in a fixed order.  Slot q = offsets[ray] + j holds sdf sample j and
+// render sample j of the ray: quadric records 2q (sdf) and 2q + 1 (normal),
+// appearance record q.  The last slot of a ray has no render sample and
+// carries invalid keys.
+struct DetRecordOut {
+    const uint32_t *offsets;
+    uint32_t *qkey, *qidx;
+    float4 *qval;
+    uint32_t *akey, *aidx;
+    float *aval;
+    uint32_t q_invalid;  // R^3
+    int cbits;           // appearance key = (parity class << cbits) | cell
+};
+
+__device__ __forceinline__ void det_put_quadric(const DetRecordOut &o, uint32_t rec, uint32_t key,
+                                                const float g[7]) {
+    o.qkey[rec] = key;
+    o.qidx[rec] = rec;
+    o.qval[2 * (int64_t)rec] = make_float4(g[0], g[1], g[2], g[3]);
+    o.qval[2 * (int64_t)rec + 1] = make_float4(g[4], g[5], g[6], 0.0f);
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
+    const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
+    const int32_t *__restrict__ ray_pixel,
+    int64_t n_rays, const float *__restrict__ qgrid, const float *__restrict__ data, int R,
+    float interval, float s, const float *__restrict__ color, const float *__restrict__ depth,
+    const float *__restrict__ gcolor, const float *__restrict__ gdepth,
+    const float4 *__restrict__ ray_state, const float *__restrict__ tlist, int K,
+    DetRecordOut o) {
+    constexpr int C = 3 * NC + 5;
+    constexpr int AP = 4 + (C + 3) / 4 * 4;
+    RaySetup ray = setup_ray(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state, tlist, K);
+    if (ray.out < 0 || ray.renders == 0) return;
+    const uint32_t base = o.offsets[ray.out];
+    const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
+    const f3 Cs = load3(color, ray.out);
+    const f3 gC = load3(gcolor, ray.out);
+    const float Ds = depth ? depth[ray.out] : 0.0f;
+    const float gD = gdepth ? gdepth[ray.out] : 0.0f;
+
+    CellPoint cp0, cp1;
+    float sdf0 = sdf_at(ray, 0, interval, half_reso, R, inv_reso, qgrid, cp0);
+    float e0 = expf(-s * sdf0);
+    float T = 1.0f;
+    f3 pre{0.0f, 0.0f, 0.0f};
+    float pre_d = 0.0f;
+    float carry = 0.0f;
+    for (uint32_t j = 0; j < ray.renders; ++j) {
+        const uint32_t q = base + j;
+        float sdf1 = sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp1);
+        float e1 = expf(-s * sdf1);
+        float inv1 = 1.0f / (1.0f + e1);
+        float praw = (e1 - e0) * inv1;
+        float a = fmaxf(praw, 0.0f);
+        float w[8], d[C];
+        SampleEval ev;
+        eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
+        const float wt = a * T;
+        pre.x += ev.rgb.x * wt;
+        pre.y += ev.rgb.y * wt;
+        pre.z += ev.rgb.z * wt;
+        pre_d += ev.t * wt;
+        float g_sdf0 = carry, g_sdf1 = 0.0f;
+        if (praw >= 0.0f) {
+            float one_over = (1.0f + e1) / (1.0f + e0);
+            float suffix = gC.x * (Cs.x - pre.x) + gC.y * (Cs.y - pre.y) + gC.z * (Cs.z - pre.z) +
+                           gD * (Ds - pre_d);
+            float here = gC.x * ev.rgb.x + gC.y * ev.rgb.y + gC.z * ev.rgb.z + gD * ev.t;
+            float da = T * here - suffix * one_over;
+            g_sdf0 += da * s * e0 * inv1;
+            g_sdf1 = -da * s * e1 * (1.0f + e0) * inv1 * inv1;
+        }
+        {
+            float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
+            quadric_sdf_grad(load_quadric(qgrid, cp0.cell), cp0.local, g_sdf0, inv_reso, gq7);
+            det_put_quadric(o, 2 * q, (uint32_t)cp0.cell, gq7);
+        }
+        carry = g_sdf1;
+        f3 grgb{gC.x * wt, gC.y * wt, gC.z * wt};
+        float dd[C];
+        f3 gn = shade_vjp<NC>(ev.normal, ray.dir, d, ev.spec, grgb, dd);
+        {
+            float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
+            quadric_normal_grad(load_quadric(qgrid, ev.rp.cell), ev.rp.local, gn, gq7);
+            det_put_quadric(o, 2 * q + 1, (uint32_t)ev.rp.cell, gq7);
+        }
+        {
+            const uint32_t cls = (uint32_t)((ev.rp.ix & 1) | ((ev.rp.iy & 1) << 1) |
+                                            ((ev.rp.iz & 1) << 2));
+            o.akey[q] = (cls << o.cbits) | (uint32_t)ev.rp.cell;
+            o.aidx[q] = q;
+            float4 *dst = reinterpret_cast<float4 *>(o.aval + (int64_t)q * AP);
+            dst[0] = make_float4(ev.rp.local.x, ev.rp.local.y, ev.rp.local.z, 0.0f);
+#pragma unroll
+            for (int c = 0; c < (AP - 4) / 4; ++c) {
+                const int c0 = 4 * c;
+                dst[1 + c] = make_float4(dd[c0], c0 + 1 < C ? dd[c0 + 1] : 0.0f,
+                                         c0 + 2 < C ? dd[c0 + 2] : 0.0f,
+                                         c0 + 3 < C ? dd[c0 + 3] : 0.0f);
+            }
+        }
+        T = T * (1.0f - a);
+        e0 = e1;
+        cp0 = cp1;
+    }
+    {
+        const uint32_t q = base + ray.renders;
+        float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
+        quadric_sdf_grad(load_quadric(qgrid, cp0.cell), cp0.local, carry, inv_reso, gq7);
+        det_put_quadric(o, 2 * q, (uint32_t)cp0.cell, gq7);
+        const float zero7[7] = {0, 0, 0, 0, 0, 0, 0};
+        det_put_quadric(o, 2 * q + 1, o.q_invalid, zero7);
+        o.akey[q] = 8u << o.cbits;
+        o.aidx[q] = q;
     }
 }
 
@@ -928,6 +1051,63 @@ int qsdvr_fused_backward_listed(const qsdvr_camera *cams, int32_t n_cams, int32_
                                K, sdf_grid, data_grid, reso, channels, interval, logistic_coef,
                                color, depth, grad_color, grad_depth, sdf_grid_grad,
                                data_grid_grad, scatter_mode, as_stream(stream));
+}
+
+size_t qsdvr_fused_det_workspace_size(int64_t n_rays, int64_t sample_capacity, int32_t channels,
+                                      int32_t reso) {
+    if (n_rays < 0 || sample_capacity < 0 || reso < 1 || reso > 512) return 0;
+    return det_workspace_bytes(n_rays, sample_capacity, channels, reso);
+}
+
+int qsdvr_fused_backward_det(const qsdvr_camera *cams, int32_t n_cams, int32_t width,
+                             int32_t height, const int32_t *ray_pixel, int64_t n_rays,
+                             const float *ray_state, const float *tlist, int32_t K,
+                             const float *sdf_grid, const float *data_grid, int32_t reso,
+                             int32_t channels, float interval, float logistic_coef,
+                             const float *color, const float *depth, const float *grad_color,
+                             const float *grad_depth, float *sdf_grid_grad,
+                             float *data_grid_grad, int64_t sample_capacity, void *workspace,
+                             size_t workspace_bytes, qsdvr_stream stream) {
+    int st = fused_check(cams, n_cams, width, height, ray_pixel, n_rays, sdf_grid, data_grid,
+                         reso, channels, interval);
+    if (st != QSDVR_OK) return st;
+    QSDVR_REQUIRE(ray_state != nullptr, "qsdvr_fused_backward_det: null ray state");
+    QSDVR_REQUIRE(color && grad_color && sdf_grid_grad && data_grid_grad,
+                  "qsdvr_fused_backward_det: null pointer argument");
+    QSDVR_REQUIRE((grad_depth == nullptr) || (depth != nullptr),
+                  "qsdvr_fused_backward_det: grad_depth needs the saved depth");
+    QSDVR_REQUIRE(tlist == nullptr || (K >= 2 && K <= 4096),
+                  "qsdvr_fused_backward_det: listed samples need 2 <= K <= 4096");
+    QSDVR_REQUIRE(reso <= 512, "qsdvr_fused_backward_det: reso %d above 512", reso);
+    QSDVR_REQUIRE(sample_capacity >= 0 && n_rays < 0xFFFFFFFFll,
+                  "qsdvr_fused_backward_det: bad sizes");
+    if (n_rays == 0) return QSDVR_OK;
+    hipStream_t s = as_stream(stream);
+    DetBuffers b;
+    st = det_carve(workspace, workspace_bytes, n_rays, sample_capacity, channels, reso, b);
+    if (st != QSDVR_OK) return st;
+    const float4 *rs = reinterpret_cast<const float4 *>(ray_state);
+    int64_t total = 0;
+    st = det_offsets(b, rs, tlist ? K : 0, s, &total);
+    if (st != QSDVR_OK) return st;
+    if (total == 0) return QSDVR_OK;
+    const uint32_t R3 = (uint32_t)((uint64_t)reso * reso * reso);
+    DetRecordOut o{b.offsets, b.qkey, b.qidx, b.qval, b.akey, b.aidx, b.aval, R3, b.cbits};
+    const float *dsave = grad_depth ? depth : nullptr;
+    const int64_t waves = fused_waves(width, height, n_cams, ray_pixel, n_rays);
+    const unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (channels == 32)
+        fused_bwd_record_kernel<9><<<blocks, kFusedBlock, 0, s>>>(
+            cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
+            data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth, rs,
+            tlist, K, o);
+    else
+        fused_bwd_record_kernel<16><<<blocks, kFusedBlock, 0, s>>>(
+            cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
+            data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth, rs,
+            tlist, K, o);
+    QSDVR_LAUNCHED();
+    return det_reduce(b, total, sdf_grid_grad, data_grid_grad, s);
 }
 
 }  // extern "C"

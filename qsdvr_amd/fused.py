@@ -21,6 +21,7 @@ from .qsdvr import CameraArgs, _ptr, _stream, _t
 
 SCATTER_TILE = 0     # tile-reduced atomics
 SCATTER_ATOMIC = 1   # one atomic per sample and channel
+SCATTER_DET = 2      # sorted-segment sums, bitwise reproducible
 
 # When set to a list, every fused kernel launch appends
 # (name, start_event, end_event) recorded on the launching stream.
@@ -101,12 +102,42 @@ class FusedRender(autograd.Function):
             gcolor = torch.zeros((n_rays, 3), dtype=torch.float32, device=dev)
         gcolor = gcolor.contiguous()
         gdep = gdepth.contiguous() if gdepth is not None else None
+        if mode == SCATTER_DET:
+            rs = torch.empty((n_rays, 8), dtype=torch.float32, device=dev)
+            _lib.call("qsdvr_ray_state", _ptr(cam_buf), n_cams, width, height, _ptr(ray_pixel),
+                      n_rays, float(interval), _ptr(rs), _stream())
+            with _timed("fused_backward"):
+                backward_det(cam_buf, n_cams, width, height, ray_pixel, rs, None, sdf_grid,
+                             data_grid, interval, s, color, depth, gcolor, gdep, gq, gd)
+            return gq, gd, None, None, None, None, None, None, None, None
         with _timed("fused_backward"):
             _lib.call("qsdvr_fused_backward", _ptr(cam_buf), n_cams, width, height,
                       _ptr(ray_pixel), n_rays, _ptr(sdf_grid), _ptr(data_grid), R, Cc, interval,
                       s, _ptr(color), _ptr(depth), _ptr(gcolor), _ptr(gdep), _ptr(gq), _ptr(gd),
                       mode, _stream())
         return gq, gd, None, None, None, None, None, None, None, None
+
+
+def backward_det(cam_buf, n_cams, width, height, ray_pixel, rs, tlist, sdf_grid, data_grid,
+                 interval, s, color, depth, gcolor, gdepth, gq, gd):
+    """Deterministic adjoint (qsdvr_fused_backward_det) on the rays of the
+    ray state ``rs``: sizes its workspace from the rays' sample counts (one
+    host synchronisation) and adds into the zeroed ``gq`` and ``gd``."""
+    n_rays = int(rs.shape[0])
+    R = int(sdf_grid.shape[0])
+    Cc = int(data_grid.shape[-1])
+    K = int(tlist.shape[1]) if tlist is not None else 0
+    cnt = rs[:, 3].contiguous().view(torch.int32).to(torch.int64)
+    per_ray = torch.where(cnt >= 2, torch.full_like(cnt, K) if K else cnt, torch.zeros_like(cnt))
+    cap = int(per_ray.sum())
+    nbytes = int(_lib.fn("qsdvr_fused_det_workspace_size")(n_rays, cap, Cc, R))
+    if nbytes == 0:
+        raise ValueError(f"deterministic scatter: unsupported sizes (reso {R} must be <= 512)")
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=sdf_grid.device)
+    _lib.call("qsdvr_fused_backward_det", _ptr(cam_buf), n_cams, width, height, _ptr(ray_pixel),
+              n_rays, _ptr(rs), _ptr(tlist), K, _ptr(sdf_grid), _ptr(data_grid), R, Cc,
+              float(interval), float(s), _ptr(color), _ptr(depth), _ptr(gcolor), _ptr(gdepth),
+              _ptr(gq), _ptr(gd), cap, _ptr(ws), nbytes, _stream())
 
 
 def render(sdf_grid, data_grid, cams, interval: float, s: float, ray_pixel=None,

@@ -24,7 +24,7 @@ import torch
 from torch import autograd
 
 from . import _lib
-from .fused import SCATTER_TILE, camera_buffer, _timed
+from .fused import SCATTER_DET, SCATTER_TILE, backward_det, camera_buffer, _timed
 from .qsdvr import _ptr, _stream, _t
 
 NBINS = 64
@@ -184,6 +184,11 @@ class ListedRender(autograd.Function):
         if gcolor is None:
             gcolor = torch.zeros((n, 3), dtype=torch.float32, device=sdf_grid.device)
         gdep = gdepth.contiguous() if gdepth is not None else None
+        if mode == SCATTER_DET:
+            with _timed("listed_backward"):
+                backward_det(cam_buf, n_cams, W, H, ray_pixel, rs, tlist, sdf_grid, data_grid,
+                             interval, s, color, depth, gcolor.contiguous(), gdep, gq, gd)
+            return (gq, gd) + none
         with _timed("listed_backward"):
             _lib.call("qsdvr_fused_backward_listed", _ptr(cam_buf), n_cams, W, H,
                       _ptr(ray_pixel), n, _ptr(rs), _ptr(tlist), K, _ptr(sdf_grid),

@@ -65,3 +65,26 @@ def test_fused_rejects_bad_arguments():
     st = lib.qsdvr_fused_forward(dummy, 1, 8, 8, None, 64, dummy, dummy, 4, 33, 0.1, 5.0,
                                  dummy, None, None)
     assert st == -1 and b"channels" in lib.qsdvr_last_error()
+
+
+def test_deterministic_backward_validates_on_the_host():
+    lib = _lib.load()
+    small = lib.qsdvr_fused_det_workspace_size(1000, 10_000, 32, 64)
+    large = lib.qsdvr_fused_det_workspace_size(1000, 100_000, 32, 64)
+    # records alone: 2 quadric (4 + 32 B) and one appearance (4 + 144 B) per slot
+    assert small >= 10_000 * (2 * 36 + 148) and large > small
+    assert lib.qsdvr_fused_det_workspace_size(1000, 100, 32, 513) == 0
+    dummy = C.c_void_p(16)
+    args = [dummy, 1, 8, 8, None, 64]
+    st = lib.qsdvr_fused_backward_det(*args, None, None, 0, dummy, dummy, 4, 32, 0.1, 5.0,
+                                      dummy, None, dummy, None, dummy, dummy, 100, dummy, 1 << 20,
+                                      None)
+    assert st == -1 and b"ray state" in lib.qsdvr_last_error()
+    st = lib.qsdvr_fused_backward_det(*args, dummy, None, 0, dummy, dummy, 4, 32, 0.1, 5.0,
+                                      dummy, None, dummy, None, dummy, dummy, 100, dummy, 64,
+                                      None)
+    assert st == -4 and b"too small" in lib.qsdvr_last_error()
+    st = lib.qsdvr_fused_backward_det(*args, dummy, None, 0, dummy, dummy, 600, 32, 0.1, 5.0,
+                                      dummy, None, dummy, None, dummy, dummy, 100, dummy, 1 << 30,
+                                      None)
+    assert st == -1 and b"512" in lib.qsdvr_last_error()
