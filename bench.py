@@ -47,6 +47,12 @@ def parse():
                     help="logistic coefficients at which the proposal path is measured "
                          "(comma list; empty: skip)")
     ap.add_argument("--proposal-train-steps", type=int, default=1000)
+    ap.add_argument("--train-steps", type=int, default=10,
+                    help="timed steps of the 256^3 random-ray training-step form per scatter "
+                         "mode (0: skip)")
+    ap.add_argument("--large-steps", type=int, default=5,
+                    help="timed steps of the 384^3 / SH 3 / 1920x1080 plain-vs-proposal "
+                         "comparison (0: skip)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run a plain device copy of this many bytes first (a counter "
                          "calibration kernel of known traffic for rocprofv3 --pmc runs)")
@@ -170,6 +176,17 @@ def main():
                                              targets, float(s_txt), args))
         log(f"proposal: {prop_results[-1]}")
 
+    # ------------------------------------------ training-step form ----
+    train = None
+    if args.train_steps > 0:
+        train = measure_training_step(args)
+        log(f"training step: {train}")
+
+    large = None
+    if args.large_steps > 0:
+        large = measure_large(args)
+        log(f"large: {large}")
+
     # ------------------------------------------------------- roofline ----
     dom = max(kern_ms, key=kern_ms.get)
     byts = algorithmic_bytes(w.R, 32, P) if dom == "fused_backward" else (
@@ -216,24 +233,28 @@ def main():
            "op_surface_rays_per_s": round(op_rays, 1) if op_rays else None,
            "proposal": prop_results,
            "kernel_ms": {k: round(v, 4) for k, v in kern_ms.items()},
+           "training_step": train, "large": large,
            "roofline": roofline, "op_surface_equivalent": op_equiv, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
 def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps=None):
     """Proposal-guided vs plain at logistic coefficient s on the same scene:
-    train the proposal on poses 0..89, then time plain and proposal
-    forward+backward steps on poses 90.. and compare PSNR on poses 95..99
-    against a dense plain render with 8x the samples (GT)."""
+    train the proposal on the first 90% of the poses, then time plain and
+    proposal forward+backward steps on the rest and compare PSNR on the last
+    5 poses against a dense plain render with 8x the samples (GT)."""
     import torch
 
     from qsdvr_amd import fused, proposal
 
     P = w.width * w.height
+    n = len(cams)
+    n_train = (9 * n) // 10
+    held = n - n_train
     steps = steps or args.steps
     prop = proposal.Proposal(seed=0)
     t0 = time.perf_counter()
-    proposal.train(prop, grid, data, cams[:90], w.interval, s,
+    proposal.train(prop, grid, data, cams[:n_train], w.interval, s,
                    steps=args.proposal_train_steps, batch=8192, lr=3e-3)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
@@ -259,12 +280,12 @@ def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps
            "train_steps": args.proposal_train_steps, "train_seconds": round(train_s, 2)}
     for name, fn in (("plain", plain), ("proposal", guided)):
         for i in range(2):
-            fn(90 + i)
+            fn(n_train + i)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for i in range(steps):
-            fn(90 + (i % 10))
+            fn(n_train + (i % held))
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / steps
@@ -272,7 +293,7 @@ def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps
         out[name + "_rays_per_s"] = round(P / (ms / 1e3), 1)
     out["speedup"] = round(out["plain_ms_per_step"] / out["proposal_ms_per_step"], 3)
     with torch.no_grad():
-        test = cams[95:100]
+        test = cams[n - 5:]
         tb = fused.camera_buffer(test)
         gt, _ = fused.render(grid, data, test, w.interval / 8, s)
         pl, _ = fused.render(grid, data, test, w.interval, s)
@@ -282,8 +303,92 @@ def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps
         out["psnr_proposal_vs_gt_db"] = round(proposal.psnr(pr, gt), 3)
         out["psnr_loss_db"] = round(out["psnr_plain_vs_gt_db"] - out["psnr_proposal_vs_gt_db"], 3)
         out["psnr_proposal_vs_plain_db"] = round(proposal.psnr(pr, pl), 3)
-        out["gt"] = f"plain renderer at 8x samples ({8 * w.n_samples} per diagonal), poses 95-99"
+        out["gt"] = (f"plain renderer at 8x samples ({8 * w.n_samples} per diagonal), "
+                     f"poses {n - 5}-{n - 1}")
     del G, D
+    torch.cuda.empty_cache()
+    return out
+
+
+def measure_large(args, s=40.0, K=48):
+    """The large-scale form: 384^3 noisy gyroid shell, SH degree 3 (53
+    channels), 50 poses at 1920x1080; plain renderer with 384 samples per
+    diagonal against the proposal with 48 fine samples, forward+backward,
+    trained and timed as the 128^3 proposal comparison."""
+    import torch
+
+    from qsdvr_amd import fused, scenes
+
+    w = scenes.WORKLOADS["large"]
+    t0 = time.time()
+    grid, data, _ = scenes.build_grids(w, seed=0, device="cuda")
+    cams = scenes.orbit_cameras(w.n_poses, w.width, w.height)
+    cam_buf = fused.camera_buffer(cams)
+    P = w.width * w.height
+    tg = torch.Generator(device="cuda").manual_seed(9)
+    targets = {p: torch.rand((P, 3), generator=tg, device="cuda")
+               for p in range((9 * w.n_poses) // 10, w.n_poses)}
+    log(f"large setup {time.time() - t0:.1f}s")
+    out = {"workload": w.name, "grid": w.R, "sh_degree": w.sh_degree, "channels": 53,
+           "width": w.width, "height": w.height, "plain_samples": w.n_samples}
+    out.update(measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=K,
+                                steps=args.large_steps))
+    del grid, data, targets
+    torch.cuda.empty_cache()
+    return out
+
+
+def measure_training_step(args):
+    """The training-step form: 256^3 union of 8 random spheres, SH degree 2,
+    each step a random batch of 65,536 rays over 100 poses at 800x800 with
+    N=256; step = draw the batch + fused forward + MSE + adjoint into zeroed
+    gradient grids.  Timed per scatter mode: deterministic (sorted-segment
+    sums) and atomic (one atomic per sample and channel); the tile mode is
+    added for reference (its wave-level reduction needs coherent rays, which
+    a random batch does not have)."""
+    import torch
+
+    from qsdvr_amd import fused, scenes
+
+    w = scenes.WORKLOADS["train"]
+    t0 = time.time()
+    grid, data, gen = scenes.build_grids(w, seed=0, device="cuda")
+    cams = scenes.orbit_cameras(w.n_poses, w.width, w.height)
+    cam_buf = fused.camera_buffer(cams)
+    log(f"training-step setup {time.time() - t0:.1f}s")
+    B = 65536
+    n_all = w.n_poses * w.width * w.height
+    G = grid.requires_grad_(True)
+    D = data.requires_grad_(True)
+    out = {"workload": w.name, "grid": w.R, "sh_degree": w.sh_degree, "rays_per_step": B,
+           "poses": w.n_poses, "samples": w.n_samples, "logistic_coef": w.s}
+    names = {fused.SCATTER_DET: "deterministic", fused.SCATTER_ATOMIC: "atomic",
+             fused.SCATTER_TILE: "tile"}
+    for mode in (fused.SCATTER_DET, fused.SCATTER_ATOMIC, fused.SCATTER_TILE):
+        g = torch.Generator(device="cuda").manual_seed(7)
+        tg = torch.Generator(device="cuda").manual_seed(8)
+
+        def step():
+            G.grad = None
+            D.grad = None
+            pick = torch.randint(0, n_all, (B,), generator=g, device="cuda").to(torch.int32)
+            target = torch.rand((B, 3), generator=tg, device="cuda")
+            c, _ = fused.FusedRender.apply(G, D, cam_buf, w.n_poses, w.width, w.height, pick,
+                                           w.interval, w.s, mode)
+            ((c - target) ** 2).mean().backward()
+
+        for _ in range(2):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.train_steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.train_steps
+        out[names[mode]] = {"ms_per_step": round(ms, 3), "rays_per_s": round(B / (ms / 1e3), 1)}
+    del G, D, grid, data
     torch.cuda.empty_cache()
     return out
 

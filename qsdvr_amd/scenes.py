@@ -104,7 +104,7 @@ def interval_for(n_samples: int) -> float:
 # Each takes float64 [...,3] normalised points and returns phi (normalised).
 
 def sdf_sphere(p, center=(0.0, 0.0, 0.0), radius_world=1.0):
-    c = torch.as_tensor(center, dtype=p.dtype)
+    c = torch.as_tensor(center, dtype=p.dtype, device=p.device)
     return torch.linalg.norm(p - c, dim=-1) - radius_world / WORLD_HALF
 
 
@@ -122,7 +122,8 @@ def make_sphere_union(seed: int = 0, n: int = 8):
     def phi(p):
         out = None
         for c, r in zip(centres, radii):
-            d = torch.linalg.norm(p - torch.as_tensor(c, dtype=p.dtype), dim=-1) - float(r)
+            d = torch.linalg.norm(p - torch.as_tensor(c, dtype=p.dtype, device=p.device),
+                                  dim=-1) - float(r)
             out = d if out is None else torch.minimum(out, d)
         return out
     return phi
@@ -193,17 +194,28 @@ def channels_for_degree(deg: int) -> int:
 
 def appearance_grid(R: int, sh_degree: int = 2, gen: torch.Generator | None = None,
                     device="cpu", channels: int | None = None) -> torch.Tensor:
+    """On the CPU the draws come from ``gen``; on a GPU device (grids of up
+    to 12 GB at 384^3 / SH 3) they come from a generator of that device
+    seeded by one draw from ``gen``, and are made in place there."""
     C = channels_for_degree(sh_degree) if channels is None else channels
     V = (R + 1) ** 3
     ncoef_active = (sh_degree + 1) ** 2
     ncoef_slots = (C - 5) // 3
+    assert ncoef_active <= ncoef_slots
+    dev = torch.device(device)
+    if dev.type != "cpu":
+        seed = int(torch.randint(0, 2 ** 62, (1,), generator=gen, dtype=torch.int64))
+        g = torch.Generator(device=dev).manual_seed(seed)
+        out = torch.zeros((V, C), dtype=torch.float32, device=dev)
+        out[:, :ncoef_active * 3].normal_(0.0, 0.1, generator=g)
+        out[:, C - 5:].uniform_(0.0, 1.0, generator=g)
+        return out
     out = torch.zeros((V, C), dtype=torch.float32)
     coef = torch.randn((V, ncoef_active * 3), generator=gen, dtype=torch.float32) * 0.1
     out[:, :ncoef_active * 3] = coef
-    assert ncoef_active <= ncoef_slots
     rest = torch.rand((V, 5), generator=gen, dtype=torch.float32)
     out[:, C - 5:] = rest
-    return out.to(device)
+    return out
 
 
 @dataclass
