@@ -222,14 +222,19 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
         for (uint32_t j = 0; j < ray.renders; ++j) {
             float e1 = expf(-s * sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp));
             float a = fmaxf((e1 - e0) / (1.0f + e1), 0.0f);
-            float w[8], d[C];
-            SampleEval ev;
-            eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
             float wt = a * T;
-            col.x += ev.rgb.x * wt;
-            col.y += ev.rgb.y * wt;
-            col.z += ev.rgb.z * wt;
-            dep += ev.t * wt;
+            // A sample of weight exactly zero (alpha clamped to 0 where the
+            // SDF rises along the ray, or T underflowed) adds nothing: its
+            // appearance gather and shading are skipped.
+            if (wt != 0.0f) {
+                float w[8], d[C];
+                SampleEval ev;
+                eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
+                col.x += ev.rgb.x * wt;
+                col.y += ev.rgb.y * wt;
+                col.z += ev.rgb.z * wt;
+                dep += ev.t * wt;
+            }
             if constexpr (BINS) {
                 const int b = min((int)(((float)j + 0.5f) / (float)ray.renders * 64.0f), 63);
                 if (b != bin) {
@@ -286,21 +291,28 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_kernel(
         float inv1 = 1.0f / (1.0f + e1);
         float praw = (e1 - e0) * inv1;
         float a = fmaxf(praw, 0.0f);
+        const float wt = a * T;
+        // The render sample is needed for the prefix (weight wt) and for
+        // dL/dalpha (when the clamp passes, times T); otherwise it adds
+        // exactly nothing and is not evaluated.
+        const bool need = praw >= 0.0f && T != 0.0f;
         float w[8], d[C];
         SampleEval ev;
-        eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
-        const float wt = a * T;
-        pre.x += ev.rgb.x * wt;
-        pre.y += ev.rgb.y * wt;
-        pre.z += ev.rgb.z * wt;
-        pre_d += ev.t * wt;
+        float here = 0.0f;
+        if (need) {
+            eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
+            pre.x += ev.rgb.x * wt;
+            pre.y += ev.rgb.y * wt;
+            pre.z += ev.rgb.z * wt;
+            pre_d += ev.t * wt;
+            here = gC.x * ev.rgb.x + gC.y * ev.rgb.y + gC.z * ev.rgb.z + gD * ev.t;
+        }
         // dL/dalpha_j = T_j (gC.rgb_j + gD t_j) - (gC.S_j + gD SD_j) / (1 - alpha_j)
         float g_sdf0 = carry, g_sdf1 = 0.0f;
         if (praw >= 0.0f) {
             float one_over = (1.0f + e1) / (1.0f + e0);
             float suffix = gC.x * (Cs.x - pre.x) + gC.y * (Cs.y - pre.y) + gC.z * (Cs.z - pre.z) +
                            gD * (Ds - pre_d);
-            float here = gC.x * ev.rgb.x + gC.y * ev.rgb.y + gC.z * ev.rgb.z + gD * ev.t;
             float da = T * here - suffix * one_over;
             g_sdf0 += da * s * e0 * inv1;
             g_sdf1 = -da * s * e1 * (1.0f + e0) * inv1 * inv1;
@@ -313,21 +325,24 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_kernel(
         }
         carry = g_sdf1;
         // rgb_j gradient -> shader adjoint -> normal and appearance grads
-        f3 grgb{gC.x * wt, gC.y * wt, gC.z * wt};
-        float dd[C];
-        f3 gn = shade_vjp<NC>(ev.normal, ray.dir, d, ev.spec, grgb, dd);
-        {
-            float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
-            quadric_normal_grad(load_quadric(qgrid, ev.rp.cell), ev.rp.local, gn, gq7);
-            scatter_quadric(gq, ev.rp.cell, gq7, 6);
-        }
+        // (all exactly zero when wt == 0)
+        if (need && wt != 0.0f) {
+            f3 grgb{gC.x * wt, gC.y * wt, gC.z * wt};
+            float dd[C];
+            f3 gn = shade_vjp<NC>(ev.normal, ray.dir, d, ev.spec, grgb, dd);
+            {
+                float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
+                quadric_normal_grad(load_quadric(qgrid, ev.rp.cell), ev.rp.local, gn, gq7);
+                scatter_quadric(gq, ev.rp.cell, gq7, 6);
+            }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int64_t row = (int64_t)ev.rp.cell + (k & 1) + ((k >> 1) & 1) * (int64_t)R +
-                                (k >> 2) * RR;
-            float *dst = gdata + row * C;
+            for (int k = 0; k < 8; ++k) {
+                const int64_t row = (int64_t)ev.rp.cell + (k & 1) + ((k >> 1) & 1) * (int64_t)R +
+                                    (k >> 2) * RR;
+                float *dst = gdata + row * C;
 #pragma unroll
-            for (int c = 0; c < C; ++c) atomic_addf(dst + c, w[k] * dd[c]);
+                for (int c = 0; c < C; ++c) atomic_addf(dst + c, w[k] * dd[c]);
+            }
         }
         T = T * (1.0f - a);
         e0 = e1;
@@ -399,20 +414,24 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
         float inv1 = 1.0f / (1.0f + e1);
         float praw = (e1 - e0) * inv1;
         float a = fmaxf(praw, 0.0f);
+        const float wt = a * T;
+        const bool need = praw >= 0.0f && T != 0.0f;  // as in fused_bwd_kernel
         float w[8], d[C];
         SampleEval ev;
-        eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
-        const float wt = a * T;
-        pre.x += ev.rgb.x * wt;
-        pre.y += ev.rgb.y * wt;
-        pre.z += ev.rgb.z * wt;
-        pre_d += ev.t * wt;
+        float here = 0.0f;
+        if (need) {
+            eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
+            pre.x += ev.rgb.x * wt;
+            pre.y += ev.rgb.y * wt;
+            pre.z += ev.rgb.z * wt;
+            pre_d += ev.t * wt;
+            here = gC.x * ev.rgb.x + gC.y * ev.rgb.y + gC.z * ev.rgb.z + gD * ev.t;
+        }
         float g_sdf0 = carry, g_sdf1 = 0.0f;
         if (praw >= 0.0f) {
             float one_over = (1.0f + e1) / (1.0f + e0);
             float suffix = gC.x * (Cs.x - pre.x) + gC.y * (Cs.y - pre.y) + gC.z * (Cs.z - pre.z) +
                            gD * (Ds - pre_d);
-            float here = gC.x * ev.rgb.x + gC.y * ev.rgb.y + gC.z * ev.rgb.z + gD * ev.t;
             float da = T * here - suffix * one_over;
             g_sdf0 += da * s * e0 * inv1;
             g_sdf1 = -da * s * e1 * (1.0f + e0) * inv1 * inv1;
@@ -423,6 +442,13 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
             det_put_quadric(o, 2 * q, (uint32_t)cp0.cell, gq7);
         }
         carry = g_sdf1;
+        if (!(need && wt != 0.0f)) {
+            // zero-weight render sample: no normal or appearance gradient
+            const float zero7[7] = {0, 0, 0, 0, 0, 0, 0};
+            det_put_quadric(o, 2 * q + 1, o.q_invalid, zero7);
+            o.akey[q] = 8u << o.cbits;
+            o.aidx[q] = q;
+        } else {
         f3 grgb{gC.x * wt, gC.y * wt, gC.z * wt};
         float dd[C];
         f3 gn = shade_vjp<NC>(ev.normal, ray.dir, d, ev.spec, grgb, dd);
@@ -445,6 +471,7 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
                                          c0 + 2 < C ? dd[c0 + 2] : 0.0f,
                                          c0 + 3 < C ? dd[c0 + 3] : 0.0f);
             }
+        }
         }
         T = T * (1.0f - a);
         e0 = e1;
@@ -662,21 +689,24 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
         for (int k = 0; k < 8; ++k) w[k] = 0.0f;
         CellPoint rp = cp0;
         float g_sdf0 = carry, g_sdf1 = 0.0f;
+        bool app = false;  // render sample j of nonzero weight: normal + appearance gradients
         if (act_r) {
             float sdf1 = sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp1);
             float e1 = expf(-s * sdf1);
             float inv1 = 1.0f / (1.0f + e1);
             float praw = (e1 - e0) * inv1;
             float a = fmaxf(praw, 0.0f);
-            float d[C];
-            SampleEval ev;
-            eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
             const float wt = a * T;
-            pre.x += ev.rgb.x * wt;
-            pre.y += ev.rgb.y * wt;
-            pre.z += ev.rgb.z * wt;
-            pre_d += ev.t * wt;
-            if (praw >= 0.0f) {
+            // the render sample is evaluated only where it adds to the prefix
+            // or to dL/dalpha; a zero-weight sample has zero gradients
+            if (praw >= 0.0f && T != 0.0f) {
+                float d[C];
+                SampleEval ev;
+                eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
+                pre.x += ev.rgb.x * wt;
+                pre.y += ev.rgb.y * wt;
+                pre.z += ev.rgb.z * wt;
+                pre_d += ev.t * wt;
                 float one_over = (1.0f + e1) / (1.0f + e0);
                 float suffix = gC.x * (Cs.x - pre.x) + gC.y * (Cs.y - pre.y) +
                                gC.z * (Cs.z - pre.z) + gD * (Ds - pre_d);
@@ -684,11 +714,25 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                 float da = T * here - suffix * one_over;
                 g_sdf0 += da * s * e0 * inv1;
                 g_sdf1 = -da * s * e1 * (1.0f + e0) * inv1 * inv1;
+                if (wt != 0.0f) {
+                    f3 grgb{gC.x * wt, gC.y * wt, gC.z * wt};
+                    f3 gn = shade_vjp<NC>(ev.normal, ray.dir, d, ev.spec, grgb, dd);
+                    quadric_normal_grad(load_quadric(qgrid, ev.rp.cell), ev.rp.local, gn, gn7);
+                    rp = ev.rp;
+                    app = true;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) w[k] = 0.0f;
+                }
+            } else if (praw >= 0.0f) {
+                // T == 0: dL/dalpha = -suffix / (1 - alpha)
+                float one_over = (1.0f + e1) / (1.0f + e0);
+                float suffix = gC.x * (Cs.x - pre.x) + gC.y * (Cs.y - pre.y) +
+                               gC.z * (Cs.z - pre.z) + gD * (Ds - pre_d);
+                float da = -suffix * one_over;
+                g_sdf0 += da * s * e0 * inv1;
+                g_sdf1 = -da * s * e1 * (1.0f + e0) * inv1 * inv1;
             }
-            f3 grgb{gC.x * wt, gC.y * wt, gC.z * wt};
-            f3 gn = shade_vjp<NC>(ev.normal, ray.dir, d, ev.spec, grgb, dd);
-            quadric_normal_grad(load_quadric(qgrid, ev.rp.cell), ev.rp.local, gn, gn7);
-            rp = ev.rp;
             T = T * (1.0f - a);
             e0 = e1;
         }
@@ -699,13 +743,14 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
             cp0 = cp1;
         }
 
+        // quadric gradients of the sdf sample and of the render sample
+        row_run_scatter<7>(gs7, act_s ? (int)sc.cell : -1, lane, gq);
+        if (__ballot(app) == 0) continue;  // no appearance or normal gradient this step
+        row_run_scatter<6>(gn7, app ? (int)rp.cell : -1, lane, gq);
         // stage this step's gradient row once, transposed: gt[c][lane]
 #pragma unroll
         for (int c = 0; c < C; ++c) gt[c * kGradStride + lane] = dd[c];
-        // quadric gradients of the sdf sample and of the render sample
-        row_run_scatter<7>(gs7, act_s ? (int)sc.cell : -1, lane, gq);
-        row_run_scatter<6>(gn7, act_r ? (int)rp.cell : -1, lane, gq);
-        bool pend_r = act_r;  // appearance gradient not yet accumulated
+        bool pend_r = app;  // appearance gradient not yet accumulated
         // Rounds: the tight box of all pending cells when it fits in kCapV
         // vertices (the common case: one round); otherwise a part of it that
         // fits (below).  Every round takes at least one pending lane, so the
