@@ -595,6 +595,41 @@ __device__ __forceinline__ void row_run_scatter(float v[7], int key, int lane,
     }
 }
 
+// Per-lane carry of quadric gradients along the ray: contributions to the
+// lane's current cell are summed in registers, and the sum goes to
+// row_run_scatter when the ray moves on to another cell (the samples of a
+// ray walk its cells in order, so a cell it has left does not come back;
+// were it to, it would only get two separate additions).  Dense samples --
+// several per cell, as the proposal places them -- then cost one set of
+// global atomics per cell instead of one per sample.
+struct QuadricCarry {
+    int cell = -1;
+    float acc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+};
+
+// Adds the first N of g[] at `cell` to the carry of the active lanes.  Must
+// be called by all 64 lanes.
+template <int N>
+__device__ __forceinline__ void carry_add(QuadricCarry &q, bool active, int cell, const float g[7],
+                                          int lane, float *__restrict__ gq) {
+    const bool flush = active && q.cell >= 0 && q.cell != cell;
+    if (__ballot(flush) != 0) {
+        float f[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) f[k] = flush ? q.acc[k] : 0.0f;
+        row_run_scatter<7>(f, flush ? q.cell : -1, lane, gq);
+    }
+    if (active) {
+        if (q.cell != cell) {
+            q.cell = cell;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) q.acc[k] = 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) q.acc[k] += g[k];
+    }
+}
+
 // Orders this wave's LDS accesses for the compiler only.  The tile kernels
 // run one wave per workgroup, and a wave's LDS instructions execute in
 // issue order, so LDS data written by one lane is visible to a later read
@@ -677,6 +712,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
     f3 pre{0.0f, 0.0f, 0.0f};
     float pre_d = 0.0f;
     float carry = 0.0f;
+    QuadricCarry qcarry;
 
     for (int j = 0; j < steps; ++j) {
         const bool act_r = live && j < (int)ray.renders;   // render sample j
@@ -744,9 +780,9 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
         }
 
         // quadric gradients of the sdf sample and of the render sample
-        row_run_scatter<7>(gs7, act_s ? (int)sc.cell : -1, lane, gq);
+        carry_add<7>(qcarry, act_s, (int)sc.cell, gs7, lane, gq);
         if (__ballot(app) == 0) continue;  // no appearance or normal gradient this step
-        row_run_scatter<6>(gn7, app ? (int)rp.cell : -1, lane, gq);
+        carry_add<6>(qcarry, app, (int)rp.cell, gn7, lane, gq);
         // stage this step's gradient row once, transposed: gt[c][lane]
 #pragma unroll
         for (int c = 0; c < C; ++c) gt[c * kGradStride + lane] = dd[c];
@@ -875,6 +911,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
             pend_r = pend_r && !in_r;
         }
     }
+    row_run_scatter<7>(qcarry.acc, qcarry.cell, lane, gq);  // the last carried cells
 }
 
 template <int NC>
