@@ -388,7 +388,31 @@ def measure_training_step(args):
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / args.train_steps
         out[names[mode]] = {"ms_per_step": round(ms, 3), "rays_per_s": round(B / (ms / 1e3), 1)}
-    del G, D, grid, data
+    # the whole GPU-resident step: the deterministic adjoint, then Adam (HIP)
+    # on the quadric and appearance grids
+    from qsdvr_amd.train import FusedTrainer
+
+    tr = FusedTrainer(D.detach(), cam_buf, w.n_poses, w.width, w.height, w.interval, w.s,
+                      sdf_grid=G.detach(), lr=1e-3, scatter_mode=fused.SCATTER_DET, seed=7)
+    tg = torch.Generator(device="cuda").manual_seed(8)
+
+    def full_step():
+        rays = tr.draw(B)
+        tr.step(rays, torch.rand((B, 3), generator=tg, device="cuda"))
+
+    for _ in range(2):
+        full_step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.train_steps):
+        full_step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.train_steps
+    out["deterministic_with_adam"] = {"ms_per_step": round(ms, 3),
+                                      "rays_per_s": round(B / (ms / 1e3), 1)}
+    del tr, G, D, grid, data
     torch.cuda.empty_cache()
     return out
 

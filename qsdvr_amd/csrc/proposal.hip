@@ -405,19 +405,40 @@ __global__ void __launch_bounds__(256) mlp_reduce_kernel(const float *__restrict
     grad[id] = s;
 }
 
+__device__ __forceinline__ void adam1(float &p, float g, float &m, float &v, float lr, float b1,
+                                      float b2, float eps, float c1, float c2) {
+    const float mm = b1 * m + (1.0f - b1) * g;
+    const float vv = b2 * v + (1.0f - b2) * g * g;
+    m = mm;
+    v = vv;
+    p -= lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+}
+
+// Bias-corrected Adam over n parameters (float4 body when `vec`, scalar
+// tail); grid-stride, 64-bit indices (appearance grids exceed 2^31 floats).
 __global__ void __launch_bounds__(256) adam_kernel(float *__restrict__ theta,
                                                    const float *__restrict__ grad,
                                                    float *__restrict__ m, float *__restrict__ v,
-                                                   int n, float lr, float b1, float b2, float eps,
-                                                   float c1, float c2) {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const float g = grad[id];
-    const float mm = b1 * m[id] + (1.0f - b1) * g;
-    const float vv = b2 * v[id] + (1.0f - b2) * g * g;
-    m[id] = mm;
-    v[id] = vv;
-    theta[id] -= lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+                                                   int64_t n, int vec, float lr, float b1, float b2,
+                                                   float eps, float c1, float c2) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n4 = vec ? n / 4 : 0;
+    for (int64_t i = tid; i < n4; i += stride) {
+        float4 p = reinterpret_cast<float4 *>(theta)[i];
+        const float4 g = reinterpret_cast<const float4 *>(grad)[i];
+        float4 mm = reinterpret_cast<float4 *>(m)[i];
+        float4 vv = reinterpret_cast<float4 *>(v)[i];
+        adam1(p.x, g.x, mm.x, vv.x, lr, b1, b2, eps, c1, c2);
+        adam1(p.y, g.y, mm.y, vv.y, lr, b1, b2, eps, c1, c2);
+        adam1(p.z, g.z, mm.z, vv.z, lr, b1, b2, eps, c1, c2);
+        adam1(p.w, g.w, mm.w, vv.w, lr, b1, b2, eps, c1, c2);
+        reinterpret_cast<float4 *>(theta)[i] = p;
+        reinterpret_cast<float4 *>(m)[i] = mm;
+        reinterpret_cast<float4 *>(v)[i] = vv;
+    }
+    for (int64_t i = 4 * n4 + tid; i < n; i += stride)
+        adam1(theta[i], grad[i], m[i], v[i], lr, b1, b2, eps, c1, c2);
 }
 
 // Inverse-CDF fine samples, one wavefront per ray (lane = bin): t = 0, t = L
@@ -563,10 +584,14 @@ int qsdvr_adam_step(float *theta, const float *grad, float *m, float *v, int64_t
                     float beta1, float beta2, float eps, int32_t step, qsdvr_stream stream) {
     QSDVR_REQUIRE(theta && grad && m && v && n > 0 && step >= 1,
                   "qsdvr_adam_step: bad arguments");
-    const float c1 = 1.0f / (1.0f - powf(beta1, (float)step));
-    const float c2 = 1.0f / (1.0f - powf(beta2, (float)step));
-    adam_kernel<<<grid_1d(n, 256), 256, 0, as_stream(stream)>>>(theta, grad, m, v, (int)n, lr,
-                                                               beta1, beta2, eps, c1, c2);
+    const float c1 = (float)(1.0 / (1.0 - pow((double)beta1, (double)step)));
+    const float c2 = (float)(1.0 / (1.0 - pow((double)beta2, (double)step)));
+    const int vec = ((reinterpret_cast<uintptr_t>(theta) | reinterpret_cast<uintptr_t>(grad) |
+                      reinterpret_cast<uintptr_t>(m) | reinterpret_cast<uintptr_t>(v)) & 15) == 0;
+    const int64_t work = vec ? (n + 3) / 4 : n;
+    const unsigned blocks = (unsigned)std::min<int64_t>((work + 255) / 256, 4096);
+    adam_kernel<<<blocks, 256, 0, as_stream(stream)>>>(theta, grad, m, v, n, vec, lr, beta1,
+                                                       beta2, eps, c1, c2);
     QSDVR_LAUNCHED();
     return QSDVR_OK;
 }
