@@ -5,6 +5,8 @@
 #include "det_scatter.h"
 #include "shade.h"
 
+#include <algorithm>
+
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -150,7 +152,11 @@ __device__ __forceinline__ int64_t run_end(const uint32_t *__restrict__ key, int
     }
 }
 
-// One wave per 64 sorted positions; it sums every run that starts there.
+// Quadric runs.  One thread per sorted position; the thread at the head of
+// a run of at most 64 records sums it alone, in order, and adds the sum to
+// its cell.  Longer runs (a cell crossed by many rays: full-frame batches)
+// are summed by the whole wave, lanes striding over the records and a
+// fixed butterfly combining them.
 __global__ void __launch_bounds__(256) det_quadric_reduce_kernel(
     const uint32_t *__restrict__ key, const uint32_t *__restrict__ idx,
     const float4 *__restrict__ qval, int64_t n, uint32_t invalid, float *__restrict__ gq) {
@@ -159,17 +165,32 @@ __global__ void __launch_bounds__(256) det_quadric_reduce_kernel(
     if (p0 >= n) return;
     const int64_t p = p0 + lane;
     const uint32_t k = p < n ? key[p] : invalid;
-    const bool boundary = p < n && (p == 0 || key[p - 1] != k);
-    const unsigned long long bmask = __ballot(boundary);
-    unsigned long long heads = __ballot(boundary && k != invalid);
-    while (heads != 0) {
-        const int h = __ffsll((long long)heads) - 1;
-        heads &= heads - 1;
+    const bool head = p < n && k != invalid && (p == 0 || key[p - 1] != k);
+    const bool long_run = head && p + kWave < n && key[p + kWave] == k;
+    if (head && !long_run) {
+        float acc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (int64_t i = p; i < n && key[i] == k; ++i) {
+            const uint32_t r = idx[i];
+            const float4 a = qval[2 * (int64_t)r], c = qval[2 * (int64_t)r + 1];
+            acc[0] += a.x;
+            acc[1] += a.y;
+            acc[2] += a.z;
+            acc[3] += a.w;
+            acc[4] += c.x;
+            acc[5] += c.y;
+            acc[6] += c.z;
+        }
+        float *dst = gq + 7 * (int64_t)k;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) dst[j] += acc[j];
+    }
+    unsigned long long longs = __ballot(long_run);
+    while (longs != 0) {
+        const int h = __ffsll((long long)longs) - 1;
+        longs &= longs - 1;
         const uint32_t hk = __shfl(k, h, kWave);
-        const unsigned long long later = bmask & ~((2ull << h) - 1);
         const int64_t start = p0 + h;
-        const int64_t end = later != 0 ? p0 + (__ffsll((long long)later) - 1)
-                                       : run_end(key, p0 + kWave, n, hk);
+        const int64_t end = run_end(key, start + kWave, n, hk);
         float acc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         for (int64_t i = start + lane; i < end; i += kWave) {
             const uint32_t r = idx[i];
@@ -213,9 +234,12 @@ __global__ void det_class_bounds_kernel(const uint32_t *__restrict__ key, int64_
     bounds[c] = (uint32_t)lo;
 }
 
-// Appearance runs of parity class `cls`: one wave per 64 sorted positions
-// sums every run that starts there, lanes over channels; the 8 weights of
-// each record come from the lane that loaded its local point.
+// Appearance runs of parity class `cls`.  Waves stride over the class's
+// sorted positions 64 at a time and sum every run that starts in their
+// chunk: a group of lanes (one lane per channel; two groups per wave when
+// C <= 32) takes one run, walks its records in order -- each lane derives
+// the record's 8 trilinear weights from its local point -- and adds the
+// 8 vertex rows to the gradient grid.
 template <int C>
 __global__ void __launch_bounds__(256) det_app_reduce_kernel(
     const uint32_t *__restrict__ key, const uint32_t *__restrict__ idx,
@@ -223,51 +247,58 @@ __global__ void __launch_bounds__(256) det_app_reduce_kernel(
     int R, float *__restrict__ gdata) {
     static_assert(C <= kWave, "one lane per channel");
     constexpr int AP = det_app_stride(C);
+    constexpr int NG = C <= kWave / 2 ? 2 : 1;  // runs in flight per wave
+    constexpr int GW = kWave / NG;
     const int64_t lo = bounds[cls], hi = bounds[cls + 1];
     const int lane = threadIdx.x % kWave;
-    const int64_t p0 =
-        lo + ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave;
-    if (p0 >= hi) return;
-    const int64_t p = p0 + lane;
-    const uint32_t k = p < hi ? key[p] : 0xFFFFFFFFu;
-    const bool boundary = p < hi && (p == lo || key[p - 1] != k);
-    unsigned long long heads = __ballot(boundary);
-    const unsigned long long bmask = heads;
+    const int grp = lane / GW, ch = lane % GW;
     const uint32_t cmask = (1u << cbits) - 1u;
     const int64_t RR = (int64_t)R * R;
-    while (heads != 0) {
-        const int h = __ffsll((long long)heads) - 1;
-        heads &= heads - 1;
-        const uint32_t hk = __shfl(k, h, kWave);
-        const unsigned long long later = bmask & ~((2ull << h) - 1);
-        const int64_t start = p0 + h;
-        const int64_t end = later != 0 ? p0 + (__ffsll((long long)later) - 1)
-                                       : run_end(key, p0 + kWave, hi, hk);
-        float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        for (int64_t c0 = start; c0 < end; c0 += kWave) {
-            // lane l fetches record c0 + l and its trilinear weights
-            const int nrec = (int)min<int64_t>(kWave, end - c0);
-            uint32_t r = 0;
-            float w[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            if (lane < nrec) {
-                r = idx[c0 + lane];
-                const float4 lp = *reinterpret_cast<const float4 *>(aval + (int64_t)r * AP);
-                trilinear_weights(f3{lp.x, lp.y, lp.z}, w);
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    for (int64_t p0 = lo + wave * kWave; p0 < hi; p0 += nwaves * kWave) {
+        const int64_t p = p0 + lane;
+        const uint32_t k = p < hi ? key[p] : 0xFFFFFFFFu;
+        const bool boundary = p < hi && (p == lo || key[p - 1] != k);
+        const unsigned long long bmask = __ballot(boundary);
+        if (bmask == 0) continue;  // inside a run that started in an earlier chunk
+        const int last = 63 - __clzll((long long)bmask);
+        const int64_t tail_end = run_end(key, p0 + kWave, hi, __shfl(k, last, kWave));
+        unsigned long long heads = bmask;
+        while (heads != 0) {
+            int hs[2] = {-1, -1};
+            hs[0] = __ffsll((long long)heads) - 1;
+            heads &= heads - 1;
+            if (NG == 2 && heads != 0) {
+                hs[1] = __ffsll((long long)heads) - 1;
+                heads &= heads - 1;
             }
-            for (int j = 0; j < nrec; ++j) {
-                const uint32_t rj = (uint32_t)__builtin_amdgcn_readlane((int)r, j);
-                const float g = lane < C ? aval[(int64_t)rj * AP + 4 + lane] : 0.0f;
+            const int my = hs[grp];
+            const uint32_t hk = __shfl(k, my < 0 ? 0 : my, kWave);
+            if (my >= 0) {
+                const unsigned long long later = bmask & ~((2ull << my) - 1);
+                const int64_t start = p0 + my;
+                const int64_t end = later != 0 ? p0 + (__ffsll((long long)later) - 1) : tail_end;
+                float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+                for (int64_t i = start; i < end; ++i) {
+                    const float *rec = aval + (int64_t)idx[i] * AP;
+                    const float4 lp = *reinterpret_cast<const float4 *>(rec);
+                    const float g = ch < C ? rec[4 + ch] : 0.0f;
+                    float w[8];
+                    trilinear_weights(f3{lp.x, lp.y, lp.z}, w);
 #pragma unroll
-                for (int v = 0; v < 8; ++v)
-                    acc[v] += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w[v]), j)) * g;
-            }
-        }
-        if (lane < C) {
-            const int64_t cell = hk & cmask;
+                    for (int v = 0; v < 8; ++v) acc[v] += w[v] * g;
+                }
+                if (ch < C) {
+                    const int64_t cell = hk & cmask;
 #pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                const int64_t row = cell + (v & 1) + ((v >> 1) & 1) * (int64_t)R + (v >> 2) * RR;
-                gdata[row * C + lane] += acc[v];
+                    for (int v = 0; v < 8; ++v) {
+                        const int64_t row =
+                            cell + (v & 1) + ((v >> 1) & 1) * (int64_t)R + (v >> 2) * RR;
+                        gdata[row * C + ch] += acc[v];
+                    }
+                }
             }
         }
     }
@@ -283,22 +314,21 @@ int det_reduce(DetBuffers &b, int64_t T, float *gq, float *gdata, hipStream_t s)
     tb = b.temp_bytes;
     QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.akey, b.akey_s, b.aidx, b.aidx_s, (size_t)T,
                                         0u, (unsigned)(b.cbits + 4), s));
-    {
-        const int64_t waves = (2 * T + kWave - 1) / kWave;
-        det_quadric_reduce_kernel<<<grid_1d(waves, 4), 256, 0, s>>>(
-            b.qkey_s, b.qidx_s, b.qval, 2 * T, (uint32_t)R3, gq);
-        QSDVR_LAUNCHED();
-    }
+    det_quadric_reduce_kernel<<<grid_1d(2 * T, 256), 256, 0, s>>>(b.qkey_s, b.qidx_s, b.qval,
+                                                                 2 * T, (uint32_t)R3, gq);
+    QSDVR_LAUNCHED();
     det_class_bounds_kernel<<<1, 64, 0, s>>>(b.akey_s, T, b.cbits, b.bounds);
     QSDVR_LAUNCHED();
-    const int64_t waves = (T + kWave - 1) / kWave;
+    // each class holds about T/8 records; the waves stride over its range
+    const int64_t chunks = (T / 8 + kWave - 1) / kWave;
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 4096));
     for (int cls = 0; cls < 8; ++cls) {
         if (b.C == 32)
-            det_app_reduce_kernel<32><<<grid_1d(waves, 4), 256, 0, s>>>(
-                b.akey_s, b.aidx_s, b.aval, b.bounds, cls, b.cbits, b.R, gdata);
+            det_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(b.akey_s, b.aidx_s, b.aval, b.bounds,
+                                                             cls, b.cbits, b.R, gdata);
         else
-            det_app_reduce_kernel<53><<<grid_1d(waves, 4), 256, 0, s>>>(
-                b.akey_s, b.aidx_s, b.aval, b.bounds, cls, b.cbits, b.R, gdata);
+            det_app_reduce_kernel<53><<<blocks, 256, 0, s>>>(b.akey_s, b.aidx_s, b.aval, b.bounds,
+                                                             cls, b.cbits, b.R, gdata);
         QSDVR_LAUNCHED();
     }
     return QSDVR_OK;
