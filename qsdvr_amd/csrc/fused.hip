@@ -504,10 +504,12 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
 // 8 x C words per sample -- spent most of its time in LDS traffic; one
 // global atomic per sample and channel was ~40x slower still.)  The 13
 // quadric gradients per sample (7 at the sdf sample's cell, 6 at the render
-// sample's) are summed over runs of equal cells along each 8-lane pixel row
-// with DPP lane shifts, and each run adds its sums with global atomics from
-// its last lane: the rays of a tile share a handful of cells, so LDS
-// atomics on a cell box serialised on the same words.  Steps whose
+// sample's) are carried per lane while its ray stays in a cell, summed over
+// equal cells across the wave (DPP runs along each 8-lane pixel row, then a
+// butterfly over rows) and added with LDS atomics into a 4x4x4-cell window
+// that goes to the grid with global atomics only when the rays leave it:
+// per-lane LDS atomics without the merge serialised on the same words, and
+// per-step global atomics held up the next step's gathers.  Steps whose
 // appearance box exceeds kCapV vertices (about 4% of the
 // uniform steps at 800x800, 12% of proposal-guided steps, mostly silhouette
 // tiles) are split into rounds of boxes that fit: slabs of the tight box
@@ -553,13 +555,13 @@ __device__ __forceinline__ void run_scan_step(float v[7], bool &f, int x) {
     }
 }
 
-// Adds the first N of v[] to quadric cell `key` (key < 0: nothing), summed
-// first over runs of equal keys along each 8-lane pixel row and then over
-// such runs in different rows; one lane per merged group issues the global
-// atomics.  Must be called by all 64 lanes.
+// Sums the first N of v[] over the lanes with equal `key` (key < 0: no
+// value): first over runs of equal keys along each 8-lane pixel row, then
+// over such runs in different rows.  Returns the key whose sum this lane
+// now holds in v[] (one lane per distinct key), or -1.  Must be called by
+// all 64 lanes.
 template <int N>
-__device__ __forceinline__ void row_run_scatter(float v[7], int key, int lane,
-                                                float *__restrict__ gq) {
+__device__ __forceinline__ int row_run_merge(float v[7], int key, int lane) {
     const int x = lane & 7;
     const int left = dpp_i<0x111>(key);
     const bool head = x == 0 || key < 0 || left != key;
@@ -587,47 +589,7 @@ __device__ __forceinline__ void row_run_scatter(float v[7], int key, int lane,
             }
         }
     }
-    if (tail >= 0) {
-        float *dst = gq + 7 * (int64_t)key;
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-            if (v[k] != 0.0f) atomic_addf(dst + k, v[k]);
-    }
-}
-
-// Per-lane carry of quadric gradients along the ray: contributions to the
-// lane's current cell are summed in registers, and the sum goes to
-// row_run_scatter when the ray moves on to another cell (the samples of a
-// ray walk its cells in order, so a cell it has left does not come back;
-// were it to, it would only get two separate additions).  Dense samples --
-// several per cell, as the proposal places them -- then cost one set of
-// global atomics per cell instead of one per sample.
-struct QuadricCarry {
-    int cell = -1;
-    float acc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-};
-
-// Adds the first N of g[] at `cell` to the carry of the active lanes.  Must
-// be called by all 64 lanes.
-template <int N>
-__device__ __forceinline__ void carry_add(QuadricCarry &q, bool active, int cell, const float g[7],
-                                          int lane, float *__restrict__ gq) {
-    const bool flush = active && q.cell >= 0 && q.cell != cell;
-    if (__ballot(flush) != 0) {
-        float f[7];
-#pragma unroll
-        for (int k = 0; k < 7; ++k) f[k] = flush ? q.acc[k] : 0.0f;
-        row_run_scatter<7>(f, flush ? q.cell : -1, lane, gq);
-    }
-    if (active) {
-        if (q.cell != cell) {
-            q.cell = cell;
-#pragma unroll
-            for (int k = 0; k < 7; ++k) q.acc[k] = 0.0f;
-        }
-#pragma unroll
-        for (int k = 0; k < N; ++k) q.acc[k] += g[k];
-    }
+    return tail;
 }
 
 // Orders this wave's LDS accesses for the compiler only.  The tile kernels
@@ -635,6 +597,127 @@ __device__ __forceinline__ void carry_add(QuadricCarry &q, bool active, int cell
 // issue order, so LDS data written by one lane is visible to a later read
 // by another lane of the wave without s_barrier.
 __device__ __forceinline__ void wave_lds_order() { __asm__ volatile("" ::: "memory"); }
+
+// Quadric gradient box: a 4x4x4-cell window of the quadric gradient grid
+// held in LDS by one wave ([coefficient][cell], 7 x 64 floats).  The merged
+// quadric sums of the tile's rays add into it with LDS float atomics; when
+// a sum falls outside, the window is written back with one global atomic
+// per nonzero entry and moved to put that cell at (1,1,1).  A step's global atomics -- whose completion
+// the next step's gathers would wait on -- become rare window write-backs.
+constexpr int kQBox = 4;
+constexpr int kQBoxCells = kQBox * kQBox * kQBox;
+static_assert(kQBoxCells == kWave, "one window cell per lane");
+
+struct QuadricBox {
+    int bx = 0, by = 0, bz = 0;
+    bool live = false;
+};
+
+__device__ __forceinline__ void qbox_writeback(float *__restrict__ qb, QuadricBox &b, int R,
+                                               int lane, float *__restrict__ gq) {
+    if (!b.live) return;
+    const int cx = b.bx + (lane & 3), cy = b.by + ((lane >> 2) & 3), cz = b.bz + (lane >> 4);
+    const bool in = cx >= 0 && cy >= 0 && cz >= 0 && cx < R && cy < R && cz < R;
+    const int64_t cell = (int64_t)cx + (int64_t)cy * R + (int64_t)cz * R * R;
+    wave_lds_order();
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const float v = qb[k * kWave + lane];
+        qb[k * kWave + lane] = 0.0f;
+        if (in && v != 0.0f) atomic_addf(gq + 7 * cell + k, v);
+    }
+    wave_lds_order();
+    b.live = false;
+}
+
+// Adds v[] (the merged sum of cell `tail` at (tx, ty, tz), tail < 0: none)
+// to the window.  Must be called by all 64 lanes.
+__device__ __forceinline__ void qbox_add(float *__restrict__ qb, QuadricBox &b, const float v[7],
+                                         int tail, int tx, int ty, int tz, int R, int lane,
+                                         float *__restrict__ gq) {
+    const bool has = tail >= 0;
+    bool in = has && b.live && (unsigned)(tx - b.bx) < (unsigned)kQBox &&
+              (unsigned)(ty - b.by) < (unsigned)kQBox && (unsigned)(tz - b.bz) < (unsigned)kQBox;
+    const unsigned long long out = __ballot(has && !in);
+    if (out != 0) {
+        const int leader = __ffsll((long long)out) - 1;
+        const int lx = __builtin_amdgcn_readlane(tx, leader);
+        const int ly = __builtin_amdgcn_readlane(ty, leader);
+        const int lz = __builtin_amdgcn_readlane(tz, leader);
+        qbox_writeback(qb, b, R, lane, gq);
+        b.bx = lx - 1;
+        b.by = ly - 1;
+        b.bz = lz - 1;
+        b.live = true;
+        in = has && (unsigned)(tx - b.bx) < (unsigned)kQBox &&
+             (unsigned)(ty - b.by) < (unsigned)kQBox && (unsigned)(tz - b.bz) < (unsigned)kQBox;
+    }
+    if (in) {
+        // LDS float atomics: the merge leaves at most a few lanes per cell
+        // (equal cells whose row runs end at different row positions)
+        const int slot = (tx - b.bx) + kQBox * (ty - b.by) + kQBox * kQBox * (tz - b.bz);
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            if (v[k] != 0.0f)
+                __hip_atomic_fetch_add(qb + k * kWave + slot, v[k], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else if (has) {  // spread wider than the window: straight to the grid
+        float *dst = gq + 7 * (int64_t)tail;
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            if (v[k] != 0.0f) atomic_addf(dst + k, v[k]);
+    }
+    wave_lds_order();
+}
+
+// Per-lane carry of quadric gradients along the ray: contributions to the
+// lane's current cell are summed in registers, and the sum goes to the
+// window when the ray moves on to another cell (the samples of a
+// ray walk its cells in order, so a cell it has left does not come back;
+// were it to, it would only get two separate additions).  Dense samples --
+// several per cell, as the proposal places them -- then cost one merge
+// per cell instead of one per sample.
+struct QuadricCarry {
+    int cell = -1;
+    int ix = 0, iy = 0, iz = 0;
+    float acc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+};
+
+// Hands the carried sums of the lanes in `flush` to the window (merged
+// first over equal cells).  Must be called by all 64 lanes.
+__device__ __forceinline__ void carry_flush(QuadricCarry &q, bool flush, float *__restrict__ qb,
+                                            QuadricBox &b, int R, int lane,
+                                            float *__restrict__ gq) {
+    float f[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) f[k] = flush ? q.acc[k] : 0.0f;
+    const int tail = row_run_merge<7>(f, flush ? q.cell : -1, lane);
+    // a merged group's cell is its lane's own carried cell
+    qbox_add(qb, b, f, tail, q.ix, q.iy, q.iz, R, lane, gq);
+}
+
+// Adds the first N of g[] at cell `cp` to the carry of the active lanes.
+// Must be called by all 64 lanes.
+template <int N>
+__device__ __forceinline__ void carry_add(QuadricCarry &q, bool active, const CellPoint &cp,
+                                          const float g[7], float *__restrict__ qb,
+                                          QuadricBox &b, int R, int lane,
+                                          float *__restrict__ gq) {
+    const bool flush = active && q.cell >= 0 && q.cell != cp.cell;
+    if (__ballot(flush) != 0) carry_flush(q, flush, qb, b, R, lane, gq);
+    if (active) {
+        if (q.cell != cp.cell) {
+            q.cell = cp.cell;
+            q.ix = cp.ix;
+            q.iy = cp.iy;
+            q.iz = cp.iz;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) q.acc[k] = 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) q.acc[k] += g[k];
+    }
+}
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -684,8 +767,10 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
     extern __shared__ float4 lds4[];
     float *gt = reinterpret_cast<float *>(lds4);  // [CP][68]: gradient rows, channel-major
     float *wt = gt + CP * kGradStride;            // [kCapV][32]: trilinear weights, vertex-major
+    float *qb = wt + kCapV * kHalf;               // [7][64]: quadric gradient window
     const int lane = threadIdx.x;
-    for (int i = lane; i < CP * kGradStride + kCapV * kHalf; i += kWave) gt[i] = 0.0f;
+    for (int i = lane; i < CP * kGradStride + kCapV * kHalf + 7 * kQBoxCells; i += kWave)
+        gt[i] = 0.0f;
 
     RaySetup ray = setup_ray<1>(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state, tlist, K);
     const bool live = ray.out >= 0 && ray.renders > 0;
@@ -713,6 +798,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
     float pre_d = 0.0f;
     float carry = 0.0f;
     QuadricCarry qcarry;
+    QuadricBox qbox;
 
     for (int j = 0; j < steps; ++j) {
         const bool act_r = live && j < (int)ray.renders;   // render sample j
@@ -780,9 +866,9 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
         }
 
         // quadric gradients of the sdf sample and of the render sample
-        carry_add<7>(qcarry, act_s, (int)sc.cell, gs7, lane, gq);
+        carry_add<7>(qcarry, act_s, sc, gs7, qb, qbox, R, lane, gq);
         if (__ballot(app) == 0) continue;  // no appearance or normal gradient this step
-        carry_add<6>(qcarry, app, (int)rp.cell, gn7, lane, gq);
+        carry_add<6>(qcarry, app, rp, gn7, qb, qbox, R, lane, gq);
         // stage this step's gradient row once, transposed: gt[c][lane]
 #pragma unroll
         for (int c = 0; c < C; ++c) gt[c * kGradStride + lane] = dd[c];
@@ -911,12 +997,14 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
             pend_r = pend_r && !in_r;
         }
     }
-    row_run_scatter<7>(qcarry.acc, qcarry.cell, lane, gq);  // the last carried cells
+    carry_flush(qcarry, qcarry.cell >= 0, qb, qbox, R, lane, gq);  // the last carried cells
+    qbox_writeback(qb, qbox, R, lane, gq);
 }
 
 template <int NC>
 constexpr size_t tile_lds_bytes() {
-    return (size_t)(tile_channels<NC>() * kGradStride + kCapV * kHalf) * sizeof(float);
+    return (size_t)(tile_channels<NC>() * kGradStride + kCapV * kHalf + 7 * kQBoxCells) *
+           sizeof(float);
 }
 
 static int64_t fused_waves(uint32_t W, uint32_t H, int n_cams, const int32_t *ray_pixel,
