@@ -59,6 +59,17 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic():
+    """HBM bytes per launch of the fused backward on this workload, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counter passes recorded next to the
+    kernel-trace summaries (None when the record is absent)."""
+    path = os.path.join(ROOT, "profiles", "fused_backward_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def algorithmic_bytes(R, C, P):
     """Compulsory HBM bytes of one fused backward launch over P rays: read the
     per-ray colour, depth and their gradients (32 B), read both grids once,
@@ -196,6 +207,10 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": None, "algorithmic_bytes_per_launch": byts,
                 "avg_launch_ms": round(kern_ms[dom], 4)}
+    traffic = measured_traffic()
+    if dom == "fused_backward" and traffic is not None:
+        roofline["traffic"] = traffic["bytes_per_launch"]
+        roofline["traffic_source"] = traffic["source"]
 
     # ------------------------------------- op-surface byte equivalent ----
     # The op-surface form of the same step moves (344 + 20 C) bytes per render
