@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--proposal-s", type=str, default="40,5",
                     help="logistic coefficients at which the proposal path is measured "
                          "(comma list; empty: skip)")
-    ap.add_argument("--proposal-train-steps", type=int, default=1000)
+    ap.add_argument("--proposal-train-steps", type=int, default=4000)
     ap.add_argument("--train-steps", type=int, default=10,
                     help="timed steps of the 256^3 random-ray training-step form per scatter "
                          "mode (0: skip)")
@@ -253,7 +253,8 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps=None):
+def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps=None,
+                     extra_gammas=()):
     """Proposal-guided vs plain at logistic coefficient s on the same scene:
     train the proposal on the first 90% of the poses, then time plain and
     proposal forward+backward steps on the rest and compare PSNR on the last
@@ -270,7 +271,7 @@ def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps
     prop = proposal.Proposal(seed=0)
     t0 = time.perf_counter()
     proposal.train(prop, grid, data, cams[:n_train], w.interval, s,
-                   steps=args.proposal_train_steps, batch=8192, lr=3e-3)
+                   steps=args.proposal_train_steps, batch=8192, lr=1e-2)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     G = grid.clone().requires_grad_(True)
@@ -292,6 +293,7 @@ def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps
         ((c - targets[p]) ** 2).mean().backward()
 
     out = {"logistic_coef": s, "fine_samples": K, "coarse_bins": 64,
+           "sampler_eps": proposal.SAMPLE_EPS, "sampler_gamma": proposal.SAMPLE_GAMMA,
            "train_steps": args.proposal_train_steps, "train_seconds": round(train_s, 2)}
     for name, fn in (("plain", plain), ("proposal", guided)):
         for i in range(2):
@@ -318,6 +320,11 @@ def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps
         out["psnr_proposal_vs_gt_db"] = round(proposal.psnr(pr, gt), 3)
         out["psnr_loss_db"] = round(out["psnr_plain_vs_gt_db"] - out["psnr_proposal_vs_gt_db"], 3)
         out["psnr_proposal_vs_plain_db"] = round(proposal.psnr(pr, pl), 3)
+        for g in extra_gammas:  # sharper samplers (same cost, PSNR only)
+            pg, _ = proposal.render_proposal(grid, data, tb, len(test), w.width, w.height,
+                                             w.interval, s, prop, K=K, gamma=g)
+            out[f"psnr_loss_db_gamma{g:g}"] = round(out["psnr_plain_vs_gt_db"] -
+                                                   proposal.psnr(pg, gt), 3)
         out["gt"] = (f"plain renderer at 8x samples ({8 * w.n_samples} per diagonal), "
                      f"poses {n - 5}-{n - 1}")
     del G, D
@@ -347,7 +354,7 @@ def measure_large(args, s=40.0, K=48):
     out = {"workload": w.name, "grid": w.R, "sh_degree": w.sh_degree, "channels": 53,
            "width": w.width, "height": w.height, "plain_samples": w.n_samples}
     out.update(measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=K,
-                                steps=args.large_steps))
+                                steps=args.large_steps, extra_gammas=(2.0,)))
     del grid, data, targets
     torch.cuda.empty_cache()
     return out

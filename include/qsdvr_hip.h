@@ -264,6 +264,16 @@ int qsdvr_fused_forward_bins(const qsdvr_camera *cams, int32_t n_cams,
                              float logistic_coef, float *color, float *depth,
                              float *bins, qsdvr_stream stream);
 
+/* The bins of qsdvr_fused_forward_bins from the SDF alone (no appearance,
+ * no colour): bins [n_rays, 64] (overwritten) of the rays whose RayCut
+ * state is ray_state [n_rays, 8] (qsdvr_ray_state).  Transmittance is
+ * multiplied in a different order than the sequential walk, so the
+ * weights agree with qsdvr_fused_forward_bins to float32 rounding.  No
+ * reference counterpart (the proposal's training targets). */
+int qsdvr_plain_bins(const float *ray_state, int64_t n_rays,
+                     const float *sdf_grid, int32_t reso, float interval,
+                     float logistic_coef, float *bins, qsdvr_stream stream);
+
 /* Per-ray state of the RayCut of every ray (same ray selection as the
  * fused entry points): ray_state [n_rays, 8] float = (origin xyz, in-cube
  * step count as uint32 bits, dir xyz, 0).  No reference counterpart. */
@@ -363,11 +373,12 @@ int qsdvr_adam_step(float *theta, const float *grad, float *m, float *v,
                     int64_t n, float lr, float beta1, float beta2, float eps,
                     int32_t step, qsdvr_stream stream);
 /* Fine sample positions tlist [n_rays, K]: t = 0, t = L and K-2 inverse-CDF
- * samples of (1-eps) w/sum(w) + eps/64 (w the composited proposal weights
- * above) over the 64 bins of [0, L], L = (count-1) * interval. */
+ * samples of (1-eps) w^gamma/sum(w^gamma) + eps/64 (w the composited
+ * proposal weights above, gamma in (0, 16] a sharpening exponent) over the
+ * 64 bins of [0, L], L = (count-1) * interval. */
 int qsdvr_proposal_sample(const float *logits, const float *ray_state,
                           int64_t n_rays, float interval, int32_t K, float eps,
-                          float *tlist, qsdvr_stream stream);
+                          float gamma, float *tlist, qsdvr_stream stream);
 
 #ifdef __cplusplus
 }

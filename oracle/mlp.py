@@ -34,7 +34,9 @@ weights over the 64 bins plus the pass-through outcome,
 with q_b the per-bin sums of the plain compositing weights.  Closed form:
   dL/dz_b = sigmoid(z_b) [ (1 - sum_{c<=b} q_c) - q_b / expm1(d_b) ].
 
-Sampler: pdf_b = (1 - eps) w_b / sum w + eps / 64; fine sdf samples are
+Sampler: pdf_b = (1 - eps) w_b^g / sum w^g + eps / 64 with the sharpening
+exponent g (g > 1 concentrates the samples where the proposal's weights
+peak, compensating the smoothness of a small network); fine sdf samples are
 t = 0, t = L and the inverse CDF of the piecewise-constant pdf at
 u_k = (k + 0.5) / (K - 2) for k < K - 2, merged in increasing order.
 
@@ -166,11 +168,13 @@ def bin_targets(w, count_render, nbins=NBINS):
     return q
 
 
-def fine_samples(z, L, K=32, eps=0.05):
+def fine_samples(z, L, K=32, eps=0.05, gamma=1.0):
     """Inverse-CDF fine sample positions t [n_rays, K] from logits z
-    [n_rays, nbins] on [0, L]."""
+    [n_rays, nbins] on [0, L]; the pdf is the proposal weights raised to
+    ``gamma``, normalised, mixed with eps of uniform."""
     n, nb = z.shape
     w, _ = proposal_weights(z.to(torch.float64))
+    w = w ** gamma
     tot = w.sum(1, keepdim=True)
     wn = torch.where(tot > 1e-30, w / tot.clamp_min(1e-300), torch.full_like(w, 1.0 / nb))
     pdf = (1 - eps) * wn + eps / nb

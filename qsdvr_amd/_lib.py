@@ -65,6 +65,7 @@ SIGNATURES = {
     "qsdvr_fused_forward_bins": ([P, I32, I32, I32, P, I64, P, P, I32, I32, F, F, P, P, P, P],
                                  C.c_int),
     "qsdvr_ray_state": ([P, I32, I32, I32, P, I64, F, P, P], C.c_int),
+    "qsdvr_plain_bins": ([P, I64, P, I32, F, F, P, P], C.c_int),
     "qsdvr_fused_forward_listed": ([P, I32, I32, I32, P, I64, P, P, I32, P, P, I32, I32, F, F,
                                     P, P, P], C.c_int),
     "qsdvr_fused_backward_listed": ([P, I32, I32, I32, P, I64, P, P, I32, P, P, I32, I32, F, F,
@@ -79,7 +80,7 @@ SIGNATURES = {
     "qsdvr_proposal_workspace_size": ([I64], SZ),
     "qsdvr_proposal_backward": ([P, P, I64, P, P, P, P, P, P, SZ, P, P, P], C.c_int),
     "qsdvr_adam_step": ([P, P, P, P, I64, F, F, F, F, I32, P], C.c_int),
-    "qsdvr_proposal_sample": ([P, P, I64, F, I32, F, P, P], C.c_int),
+    "qsdvr_proposal_sample": ([P, P, I64, F, I32, F, F, P, P], C.c_int),
 }
 
 _lib = None

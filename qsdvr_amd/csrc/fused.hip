@@ -22,6 +22,8 @@
 
 #include <limits.h>
 
+#include <algorithm>
+
 namespace qsdvr {
 
 constexpr int kFusedBlock = 256;
@@ -1060,6 +1062,73 @@ __global__ void __launch_bounds__(256) ray_state_kernel(const qsdvr_camera *__re
     out[2 * i + 1] = b;
 }
 
+// Compositing weights of the plain uniform renderer summed into 64 equal
+// bins per ray, from the SDF alone (the proposal's training targets): one
+// wavefront per ray, lanes over 64 consecutive render samples at a time.
+// alpha_j needs sdf samples j and j+1 (the next lane's; lane 63 evaluates its
+// own), T_j is the carried transmittance times an exclusive product scan of
+// (1 - alpha) over the lanes, and each lane adds w_j = alpha_j T_j to its
+// bin with an LDS atomic.  Bins as in fused_fwd_kernel<.., true>; T is
+// multiplied in scan order instead of sample order, so the weights agree
+// with the sequential walk to float32 rounding.
+__global__ void __launch_bounds__(256) plain_bins_kernel(const float4 *__restrict__ ray_state,
+                                                         int64_t n_rays,
+                                                         const float *__restrict__ qgrid, int R,
+                                                         float interval, float s,
+                                                         float *__restrict__ bins) {
+    __shared__ float lb[256 / kWave][64];
+    const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+    const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / kWave) + wv; r < n_rays; r += nwaves) {
+        const float4 a4 = ray_state[2 * r], b4 = ray_state[2 * r + 1];
+        const uint32_t count = __float_as_uint(a4.w);
+        lb[wv][lane] = 0.0f;
+        __builtin_amdgcn_wave_barrier();
+        if (count >= 2) {
+            const f3 ori{a4.x, a4.y, a4.z}, dir{b4.x, b4.y, b4.z};
+            const uint32_t renders = count - 1;
+            float T = 1.0f;
+            for (uint32_t c0 = 0; c0 < renders; c0 += kWave) {
+                const uint32_t j = c0 + lane;
+                const bool valid = j < renders;
+                float e = 0.0f, e_next = 0.0f;
+                if (j <= renders) {
+                    CellPoint cp = sdf_sample(ori, dir, j, interval, half_reso, R);
+                    clamp_cell(cp, R);
+                    e = expf(-s * quadric_sdf(load_quadric(qgrid, cp.cell), cp.local, inv_reso));
+                }
+                e_next = __shfl_down(e, 1, kWave);
+                if (lane == kWave - 1 && j + 1 <= renders) {
+                    CellPoint cp = sdf_sample(ori, dir, j + 1, interval, half_reso, R);
+                    clamp_cell(cp, R);
+                    e_next = expf(-s * quadric_sdf(load_quadric(qgrid, cp.cell), cp.local, inv_reso));
+                }
+                const float a = valid ? fmaxf((e_next - e) / (1.0f + e_next), 0.0f) : 0.0f;
+                // inclusive product scan of (1 - a) over the lanes
+                float p = 1.0f - a;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const float u = __shfl_up(p, o, kWave);
+                    if (lane >= o) p *= u;
+                }
+                const float up = __shfl_up(p, 1, kWave);  // every lane shuffles
+                const float excl = lane == 0 ? 1.0f : up;
+                const float w = a * (T * excl);
+                if (valid && w != 0.0f) {
+                    const int b = min((int)(((float)j + 0.5f) / (float)renders * 64.0f), 63);
+                    __hip_atomic_fetch_add(&lb[wv][b], w, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                T *= __shfl(p, kWave - 1, kWave);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        bins[r * 64 + lane] = lb[wv][lane];
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 static int fused_forward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t width,
                               int32_t height, const int32_t *ray_pixel, int64_t n_rays,
                               const float *ray_state, const float *tlist, int32_t K,
@@ -1191,6 +1260,21 @@ int qsdvr_ray_state(const qsdvr_camera *cams, int32_t n_cams, int32_t width, int
     ray_state_kernel<<<grid_1d(n_rays, 256), 256, 0, as_stream(stream)>>>(
         cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, interval,
         reinterpret_cast<float4 *>(ray_state));
+    QSDVR_LAUNCHED();
+    return QSDVR_OK;
+}
+
+int qsdvr_plain_bins(const float *ray_state, int64_t n_rays, const float *sdf_grid,
+                     int32_t reso, float interval, float logistic_coef, float *bins,
+                     qsdvr_stream stream) {
+    QSDVR_REQUIRE(n_rays >= 0 && reso > 0 && reso <= 1290 && interval > 0.0f,
+                  "qsdvr_plain_bins: bad sizes");
+    if (n_rays == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(ray_state && sdf_grid && bins, "qsdvr_plain_bins: null pointer argument");
+    const int64_t blocks = std::min<int64_t>((n_rays + 3) / 4, 8192);
+    plain_bins_kernel<<<(unsigned)blocks, 256, 0, as_stream(stream)>>>(
+        reinterpret_cast<const float4 *>(ray_state), n_rays, sdf_grid, reso, interval,
+        logistic_coef, bins);
     QSDVR_LAUNCHED();
     return QSDVR_OK;
 }

@@ -340,45 +340,65 @@ __global__ void __launch_bounds__(256) mlp_bwd_point_kernel(
 }
 
 // Partial weight gradients of one layer over a chunk of points:
-// part[o * 64 + i] = sum_pt dpre[pt][o] x[pt][i], part[4096 + o] = sum dpre.
-// width = 64 (hidden layers) or 1 (output layer, dpre = dz).
-constexpr int kWgradChunk = 4096;
+// part[o * 64 + i] = sum_pt dpre[pt][o] x[pt][i], part[4096 + o] = sum dpre,
+// as a (WIDTH x chunk) x (chunk x 64) product on the matrix cores in float32
+// (v_mfma_f32_16x16x4_f32, one wavefront per chunk, K = 4 points per step).
+// Lane t supplies A[t % 16][t / 16] = dpre[pt0 + t / 16][16 mt + t % 16] and
+// B[t / 16][t % 16] = x[pt0 + t / 16][16 nt + t % 16] and holds
+// D[4 (t / 16) + i][t % 16].  WIDTH = 64 (hidden layers) or 1 (the output
+// layer: dpre = dz [npts], one 16-row tile whose rows 1..15 are zero).
+constexpr int kWgradChunk = 1024;
 
-__global__ void __launch_bounds__(256) mlp_wgrad_kernel(const float *__restrict__ dpre, int width,
-                                                        const float *__restrict__ x, int64_t npts,
-                                                        float *__restrict__ part) {
-    __shared__ float sd[64][65], sx[64][65];
-    const int t = threadIdx.x, o = t >> 2, i0 = (t & 3) * 16;
-    float acc[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-    float bsum = 0.0f;
+template <int WIDTH>
+__global__ void __launch_bounds__(kWave) mlp_wgrad_kernel(const float *__restrict__ dpre,
+                                                          const float *__restrict__ x,
+                                                          int64_t npts, float *__restrict__ part) {
+    constexpr int MT = WIDTH == 64 ? 4 : 1;
+    const int t = threadIdx.x, r = t >> 4, m = t & 15;
     const int64_t c0 = (int64_t)blockIdx.x * kWgradChunk;
     const int64_t c1 = min(c0 + kWgradChunk, npts);
-    for (int64_t s0 = c0; s0 < c1; s0 += 64) {
-        __syncthreads();
-        for (int k = t; k < 64 * 64; k += 256) {
-            const int pp = k >> 6, ff = k & 63;
-            const int64_t pt = s0 + pp;
-            const bool ok = pt < c1;
-            sx[pp][ff] = ok ? x[pt * 64 + ff] : 0.0f;
-            sd[pp][ff] = (ok && ff < width) ? dpre[pt * width + ff] : 0.0f;
-        }
-        __syncthreads();
-        if (o < width) {
-            for (int pp = 0; pp < 64; ++pp) {
-                const float d = sd[pp][o];
-                bsum += d;
+    v4f acc[MT][4];
+    float bsum[MT];
 #pragma unroll
-                for (int i = 0; i < 16; ++i) acc[i] += d * sx[pp][i0 + i];
-            }
+    for (int mt = 0; mt < MT; ++mt) {
+        bsum[mt] = 0.0f;
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    for (int64_t p0 = c0; p0 < c1; p0 += 4) {
+        const int64_t pt = p0 + r;
+        const bool ok = pt < c1;
+        float av[MT], bv[4];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            if (WIDTH == 64) av[mt] = ok ? dpre[pt * 64 + 16 * mt + m] : 0.0f;
+            else av[mt] = ok && m == 0 ? dpre[pt] : 0.0f;
+            bsum[mt] += av[mt];
         }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) bv[nt] = ok ? x[pt * 64 + 16 * nt + m] : 0.0f;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[mt], bv[nt], acc[mt][nt], 0,
+                                                                   0, 0);
     }
     float *dst = part + (int64_t)blockIdx.x * (64 * 64 + 64);
-    if (o < width) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) dst[o * 64 + i0 + i] = acc[i];
-        if ((t & 3) == 0) dst[4096 + o] = bsum;
+    for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int o = 16 * mt + 4 * r + i;
+                if (o < WIDTH) dst[o * 64 + 16 * nt + m] = acc[mt][nt][i];
+            }
+        // bias: the column sums, gathered over the four point rows of a step
+        float v = bsum[mt];
+        v += __shfl_xor(v, 16, kWave);
+        v += __shfl_xor(v, 32, kWave);
+        if (t < 16 && 16 * mt + t < WIDTH) dst[4096 + 16 * mt + t] = v;
     }
 }
 
@@ -443,7 +463,8 @@ __global__ void __launch_bounds__(256) adam_kernel(float *__restrict__ theta,
 
 // Inverse-CDF fine samples, one wavefront per ray (lane = bin): t = 0, t = L
 // and K-2 interior samples at u = (k + 0.5)/(K-2) of the mixture pdf
-// (1-eps) w/sum(w) + eps/64, w the composited proposal weights of the bins.
+// (1-eps) w^g/sum(w^g) + eps/64, w the composited proposal weights of the
+// bins and g the sharpening exponent.
 __device__ __forceinline__ float wave_scan_incl(float v, int lane) {
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
@@ -456,7 +477,8 @@ __device__ __forceinline__ float wave_scan_incl(float v, int lane) {
 __global__ void __launch_bounds__(256) sampler_kernel(const float *__restrict__ logits,
                                                       const float4 *__restrict__ ray_state,
                                                       int64_t n_rays, float interval, int K,
-                                                      float eps, float *__restrict__ tlist) {
+                                                      float eps, float gamma,
+                                                      float *__restrict__ tlist) {
     __shared__ float s_lo[4][kBins], s_pdf[4][kBins];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int64_t r = (int64_t)blockIdx.x * 4 + wv;
@@ -471,7 +493,8 @@ __global__ void __launch_bounds__(256) sampler_kernel(const float *__restrict__ 
     const float z = logits[r * kBins + lane];
     const float d = z > 15.0f ? z : log1pf(expf(z));
     const float S = wave_scan_incl(d, lane);
-    const float wb = expf(-(S - d)) * -expm1f(-d);
+    const float w1 = expf(-(S - d)) * -expm1f(-d);
+    const float wb = gamma == 1.0f ? w1 : (gamma == 2.0f ? w1 * w1 : powf(w1, gamma));
     float tot = wb;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, kWave);
@@ -570,10 +593,10 @@ int qsdvr_proposal_backward(const float *logits, const float *target, int64_t n_
     mlp_bwd_point_kernel<<<(unsigned)((npts + 255) / 256), 256, 0, s>>>(
         logits, target, n_rays, theta, x1, x2, h3, dp1, dp2, dp3, dz, loss);
     QSDVR_LAUNCHED();
-    mlp_wgrad_kernel<<<(unsigned)nch, 256, 0, s>>>(dp1, 64, x0, npts, pt1);
-    mlp_wgrad_kernel<<<(unsigned)nch, 256, 0, s>>>(dp2, 64, x1, npts, pt2);
-    mlp_wgrad_kernel<<<(unsigned)nch, 256, 0, s>>>(dp3, 64, x2, npts, pt3);
-    mlp_wgrad_kernel<<<(unsigned)nch, 256, 0, s>>>(dz, 1, h3, npts, pt4);
+    mlp_wgrad_kernel<64><<<(unsigned)nch, kWave, 0, s>>>(dp1, x0, npts, pt1);
+    mlp_wgrad_kernel<64><<<(unsigned)nch, kWave, 0, s>>>(dp2, x1, npts, pt2);
+    mlp_wgrad_kernel<64><<<(unsigned)nch, kWave, 0, s>>>(dp3, x2, npts, pt3);
+    mlp_wgrad_kernel<1><<<(unsigned)nch, kWave, 0, s>>>(dz, h3, npts, pt4);
     QSDVR_LAUNCHED();
     mlp_reduce_kernel<<<grid_1d(kParams, 256), 256, 0, s>>>(pt1, pt2, pt3, pt4, (int)nch, grad);
     QSDVR_LAUNCHED();
@@ -597,14 +620,16 @@ int qsdvr_adam_step(float *theta, const float *grad, float *m, float *v, int64_t
 }
 
 int qsdvr_proposal_sample(const float *logits, const float *ray_state, int64_t n_rays,
-                          float interval, int32_t K, float eps, float *tlist,
+                          float interval, int32_t K, float eps, float gamma, float *tlist,
                           qsdvr_stream stream) {
-    QSDVR_REQUIRE(n_rays >= 0 && K >= 3 && K <= 4096 && eps >= 0.0f && eps <= 1.0f,
+    QSDVR_REQUIRE(n_rays >= 0 && K >= 3 && K <= 4096 && eps >= 0.0f && eps <= 1.0f &&
+                      gamma > 0.0f && gamma <= 16.0f,
                   "qsdvr_proposal_sample: bad sizes");
     if (n_rays == 0) return QSDVR_OK;
     QSDVR_REQUIRE(logits && ray_state && tlist, "qsdvr_proposal_sample: null pointer argument");
     sampler_kernel<<<(unsigned)((n_rays + 3) / 4), 256, 0, as_stream(stream)>>>(
-        logits, reinterpret_cast<const float4 *>(ray_state), n_rays, interval, K, eps, tlist);
+        logits, reinterpret_cast<const float4 *>(ray_state), n_rays, interval, K, eps, gamma,
+        tlist);
     QSDVR_LAUNCHED();
     return QSDVR_OK;
 }

@@ -80,13 +80,14 @@ def test_sampler_matches_oracle():
     _, rs = _state(cams, interval)
     n = rs.shape[0]
     z = torch.randn((n, 64), generator=torch.Generator().manual_seed(6)) * 3
-    t = proposal.sample(z.cuda(), rs, interval, K=32, eps=0.05).cpu().double()
     cnt = proposal.ray_counts(rs).cpu().long()
     L = torch.clamp(cnt - 1, min=0).double() * np.float32(interval)
-    ref = om.fine_samples(z, L, K=32, eps=0.05)
-    err = ((t - ref).abs().max(1).values / L.clamp_min(1e-9))
-    assert float(err[cnt >= 2].max()) < 2e-5
-    assert torch.all(t[cnt >= 2][:, 1:] >= t[cnt >= 2][:, :-1])
+    for eps, gamma in ((0.05, 1.0), (0.0, 2.0), (0.1, 1.5)):
+        t = proposal.sample(z.cuda(), rs, interval, K=32, eps=eps, gamma=gamma).cpu().double()
+        ref = om.fine_samples(z, L, K=32, eps=eps, gamma=gamma)
+        err = ((t - ref).abs().max(1).values / L.clamp_min(1e-9))
+        assert float(err[cnt >= 2].max()) < 2e-5, (eps, gamma)
+        assert torch.all(t[cnt >= 2][:, 1:] >= t[cnt >= 2][:, :-1])
 
 
 def _listed_lists(rs, t, R):
@@ -167,6 +168,11 @@ def test_listed_uniform_is_plain_and_bins():
     q = om.bin_targets(wts, rc)
     P = w.width * w.height
     assert_close(bins[:P].cpu(), q, 1e-4, "bin targets")
+    # the SDF-only bins kernel gives the same targets (T multiplied in scan order)
+    sdf_bins = proposal.plain_bins_sdf(G, rs, w.interval, w.s)
+    assert_close(sdf_bins.cpu(), bins.cpu(), 1e-5, "SDF-only bins vs fused-forward bins")
+    err = (sdf_bins - bins).abs().max().item()
+    assert err <= 1e-5, err
 
 
 def _rgb_sdf(grid, data, smp, w):
@@ -182,8 +188,10 @@ def _rgb_sdf(grid, data, smp, w):
 
 
 def test_training_gains_over_uniform_sampling():
-    """Trained proposal (32 fine samples) vs the plain uniform renderer at the
-    same 32 samples and vs the untrained network, on held-out poses."""
+    """Trained proposal (32 fine samples) against the plain renderer at 192
+    samples (at most 0.1 dB worse), the plain uniform renderer at the same 32
+    samples and the untrained network with the same sampler, on held-out
+    poses."""
     s = 40.0
     w = scenes.Workload("gain", 64, 2, 96, 96, 192, 12, "torus", 0.005, s=s)
     grid, data, _ = scenes.build_grids(w, seed=0, device="cuda")
@@ -198,9 +206,11 @@ def test_training_gains_over_uniform_sampling():
     W, H, n = w.width, w.height, len(test_cams)
     gt, _ = fused.render(grid, data, test_cams, w.interval / 8, s)
     u32, _ = fused.render(grid, data, test_cams, 2 * np.sqrt(3) / 32, s)
+    p192, _ = fused.render(grid, data, test_cams, w.interval, s)
     c_tr, _ = proposal.render_proposal(grid, data, buf, n, W, H, w.interval, s, prop, K=32)
     c_un, _ = proposal.render_proposal(grid, data, buf, n, W, H, w.interval, s, untrained, K=32)
-    p_tr, p_u, p_un = (proposal.psnr(c, gt) for c in (c_tr, u32, c_un))
-    print(f"PSNR vs GT: trained proposal {p_tr:.2f} dB, uniform-32 {p_u:.2f} dB, "
-          f"untrained {p_un:.2f} dB")
-    assert p_tr > p_u + 3.0 and p_tr > p_un + 3.0
+    p_tr, p_u, p_un, p_pl = (proposal.psnr(c, gt) for c in (c_tr, u32, c_un, p192))
+    print(f"PSNR vs GT: trained proposal {p_tr:.2f} dB, plain-192 {p_pl:.2f} dB, "
+          f"uniform-32 {p_u:.2f} dB, untrained {p_un:.2f} dB")
+    assert p_tr >= p_pl - 0.1
+    assert p_tr > p_u + 3.0 and p_tr > p_un + 1.0

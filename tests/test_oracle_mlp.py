@@ -84,6 +84,22 @@ def test_sampler_invariants():
     assert int(inside) >= 27 and torch.all(t[1:] > t[:-1])
 
 
+def test_sampler_sharpening_exponent():
+    """Two surfaces of composited weight 0.2 (bin 5) and 0.6 (bin 20): the
+    interior samples split 1:3 at gamma = 1 and 1:9 at gamma = 2."""
+    def logit(d):
+        return math.log(math.expm1(d))
+    z = torch.full((1, 64), -60.0, dtype=torch.float64)
+    z[0, 5] = logit(-math.log(1 - 0.2))
+    z[0, 20] = logit(math.log(4.0))
+    w, _ = om.proposal_weights(z)
+    assert abs(float(w[0, 5]) - 0.2) < 1e-9 and abs(float(w[0, 20]) - 0.6) < 1e-9
+    for gamma, n5 in ((1.0, 7), (2.0, 3)):
+        t = om.fine_samples(z, torch.tensor([64.0]), K=32, eps=0.0, gamma=gamma)[0]
+        assert int(((t >= 5) & (t < 6)).sum()) == n5
+        assert int(((t >= 20) & (t < 21)).sum()) == 30 - n5
+
+
 def test_composite_t_matches_list_compositing():
     from oracle import render as orc
     g = torch.Generator().manual_seed(4)
