@@ -215,6 +215,41 @@ int qsdvr_ray_aggregate_backward(const float *grad_out, const float *rgb,
                                  int64_t render_total, float logistic_coef,
                                  qsdvr_stream stream);
 
+/* ---------------------------------------------- sorted-segment scatters --- */
+
+/* Sorted-segment forms of the three accumulating backward ops, with the same
+ * arguments and the same result (added into the caller-zeroed grid) as
+ * qsdvr_grid_interpolation_backward (kernel/render.cu:325-331),
+ * qsdvr_sample_sdf_backward (kernel/surface.cu:352-359) and
+ * qsdvr_sample_normal_backward (kernel/surface.cu:369-375).  The sample
+ * indices are radix-sorted by cell and every cell's samples are summed in
+ * sample order before one read-modify-write of the cell's rows (the
+ * appearance in 8 cell-parity passes), instead of one float atomic per
+ * sample and coefficient: bitwise reproducible.  Workspace:
+ * qsdvr_sorted_scatter_workspace_size(n, cells) bytes with cells = R^3
+ * (cell_count for the quadric ops, (vertex_reso - 1)^3 for the
+ * interpolation).  GridInterpolation: channels 32 or 53 and an even cell
+ * resolution (with the stride-R vertex rows of the reference, cells of one
+ * parity class would share a row at an odd resolution); the atomic op
+ * covers the rest. */
+size_t qsdvr_sorted_scatter_workspace_size(int64_t n, int64_t cell_count);
+int qsdvr_grid_interpolation_backward_sorted(float *grad_grid, const float *point,
+                                             const int32_t *index,
+                                             const float *grad_out, int64_t n,
+                                             int32_t vertex_reso, int32_t channels,
+                                             void *workspace, size_t workspace_bytes,
+                                             qsdvr_stream stream);
+int qsdvr_sample_sdf_backward_sorted(const float *grad_out, const float *sdf_grid,
+                                     const float *point, const int32_t *index,
+                                     float *grad_grid, int64_t n, int64_t cell_count,
+                                     float reso, void *workspace,
+                                     size_t workspace_bytes, qsdvr_stream stream);
+int qsdvr_sample_normal_backward_sorted(const float *grad_out, const float *sdf_grid,
+                                        const float *point, const int32_t *index,
+                                        float *grad_grid, int64_t n,
+                                        int64_t cell_count, void *workspace,
+                                        size_t workspace_bytes, qsdvr_stream stream);
+
 /* ------------------------------------------------------------- fused --- */
 
 /* No reference counterpart.  The whole plain render of kernel/camera.cu
@@ -321,7 +356,7 @@ int qsdvr_fused_backward_listed(const qsdvr_camera *cams, int32_t n_cams,
  * qsdvr_ray_state.  sample_capacity bounds the total sdf samples of all
  * rays (sum over rays of count, or of K when listed, for rays with count
  * >= 2); the workspace holds qsdvr_fused_det_workspace_size() bytes for it.
- * reso <= 512.  Synchronises the host once (reads back the sample total);
+ * reso even and <= 512.  Synchronises the host once (reads back the sample total);
  * QSDVR_EWORKSPACE if the total exceeds sample_capacity. */
 size_t qsdvr_fused_det_workspace_size(int64_t n_rays, int64_t sample_capacity,
                                       int32_t channels, int32_t reso);

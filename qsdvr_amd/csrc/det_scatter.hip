@@ -3,6 +3,7 @@
 // equal keys in a fixed order (one wavefront per run).  See det_scatter.h
 // for the record formats and why the appearance runs go in 8 parity passes.
 #include "det_scatter.h"
+#include "quadric.h"
 #include "shade.h"
 
 #include <algorithm>
@@ -67,7 +68,8 @@ size_t det_workspace_bytes(int64_t n_rays, int64_t cap, int C, int R) {
 }
 
 int det_carve(void *ws, size_t bytes, int64_t n_rays, int64_t cap, int C, int R, DetBuffers &b) {
-    QSDVR_REQUIRE(R >= 1 && R <= 512, "deterministic scatter: reso %d out of range (1..512)", R);
+    QSDVR_REQUIRE(R >= 2 && R <= 512 && R % 2 == 0,
+                  "deterministic scatter: reso %d must be even and in 2..512", R);
     if (2 * cap >= 0xFFFFFFFFll) {
         set_error("deterministic scatter: %lld sample slots exceed the 32-bit record index",
                   (long long)cap);
@@ -152,14 +154,48 @@ __device__ __forceinline__ int64_t run_end(const uint32_t *__restrict__ key, int
     }
 }
 
+// Where a quadric run's record r gets its 7 coefficient gradients from:
+// the adjoint's precomputed records, or (op surface) the closed forms of the
+// SampleSDF / SampleNormal adjoints at the sample's point and upstream
+// gradient, with the cell's quadric.
+struct QuadricRecords {
+    const float4 *qval;
+    __device__ __forceinline__ void add(uint32_t r, uint32_t, float acc[7]) const {
+        const float4 a = qval[2 * (int64_t)r], c = qval[2 * (int64_t)r + 1];
+        acc[0] += a.x;
+        acc[1] += a.y;
+        acc[2] += a.z;
+        acc[3] += a.w;
+        acc[4] += c.x;
+        acc[5] += c.y;
+        acc[6] += c.z;
+    }
+};
+
+struct SdfSamples {
+    const float *grid, *pts, *up;
+    float inv_reso;
+    __device__ __forceinline__ void add(uint32_t r, uint32_t cell, float acc[7]) const {
+        quadric_sdf_grad(load_quadric(grid, cell), load3(pts, r), up[r], inv_reso, acc);
+    }
+};
+
+struct NormalSamples {
+    const float *grid, *pts, *up;
+    __device__ __forceinline__ void add(uint32_t r, uint32_t cell, float acc[7]) const {
+        quadric_normal_grad(load_quadric(grid, cell), load3(pts, r), load3(up, r), acc);
+    }
+};
+
 // Quadric runs.  One thread per sorted position; the thread at the head of
 // a run of at most 64 records sums it alone, in order, and adds the sum to
-// its cell.  Longer runs (a cell crossed by many rays: full-frame batches)
-// are summed by the whole wave, lanes striding over the records and a
-// fixed butterfly combining them.
+// its cell.  Longer runs (a cell crossed by many rays: full frames, and the
+// op surface's per-pose sample lists) are summed by the whole wave, lanes
+// striding over the records and a fixed butterfly combining them.
+template <class Src>
 __global__ void __launch_bounds__(256) det_quadric_reduce_kernel(
-    const uint32_t *__restrict__ key, const uint32_t *__restrict__ idx,
-    const float4 *__restrict__ qval, int64_t n, uint32_t invalid, float *__restrict__ gq) {
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ idx, Src src, int64_t n,
+    uint32_t invalid, float *__restrict__ gq) {
     const int lane = threadIdx.x % kWave;
     const int64_t p0 = ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave;
     if (p0 >= n) return;
@@ -169,17 +205,7 @@ __global__ void __launch_bounds__(256) det_quadric_reduce_kernel(
     const bool long_run = head && p + kWave < n && key[p + kWave] == k;
     if (head && !long_run) {
         float acc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        for (int64_t i = p; i < n && key[i] == k; ++i) {
-            const uint32_t r = idx[i];
-            const float4 a = qval[2 * (int64_t)r], c = qval[2 * (int64_t)r + 1];
-            acc[0] += a.x;
-            acc[1] += a.y;
-            acc[2] += a.z;
-            acc[3] += a.w;
-            acc[4] += c.x;
-            acc[5] += c.y;
-            acc[6] += c.z;
-        }
+        for (int64_t i = p; i < n && key[i] == k; ++i) src.add(idx[i], k, acc);
         float *dst = gq + 7 * (int64_t)k;
 #pragma unroll
         for (int j = 0; j < 7; ++j) dst[j] += acc[j];
@@ -192,17 +218,7 @@ __global__ void __launch_bounds__(256) det_quadric_reduce_kernel(
         const int64_t start = p0 + h;
         const int64_t end = run_end(key, start + kWave, n, hk);
         float acc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        for (int64_t i = start + lane; i < end; i += kWave) {
-            const uint32_t r = idx[i];
-            const float4 a = qval[2 * (int64_t)r], c = qval[2 * (int64_t)r + 1];
-            acc[0] += a.x;
-            acc[1] += a.y;
-            acc[2] += a.z;
-            acc[3] += a.w;
-            acc[4] += c.x;
-            acc[5] += c.y;
-            acc[6] += c.z;
-        }
+        for (int64_t i = start + lane; i < end; i += kWave) src.add(idx[i], hk, acc);
 #pragma unroll
         for (int o = kWave / 2; o > 0; o >>= 1) {
 #pragma unroll
@@ -240,13 +256,35 @@ __global__ void det_class_bounds_kernel(const uint32_t *__restrict__ key, int64_
 // C <= 32) takes one run, walks its records in order -- each lane derives
 // the record's 8 trilinear weights from its local point -- and adds the
 // 8 vertex rows to the gradient grid.
+// Appearance record r: local point and gradient row, from the adjoint's
+// records or (op surface) GridInterpolationBackward's point list and
+// upstream gradient rows.
 template <int C>
+struct AppRecords {
+    const float *aval;
+    __device__ __forceinline__ f3 local(uint32_t r) const {
+        const float4 lp = *reinterpret_cast<const float4 *>(aval + (int64_t)r * det_app_stride(C));
+        return f3{lp.x, lp.y, lp.z};
+    }
+    __device__ __forceinline__ float grad(uint32_t r, int ch) const {
+        return aval[(int64_t)r * det_app_stride(C) + 4 + ch];
+    }
+};
+
+template <int C>
+struct InterpSamples {
+    const float *pts, *up;
+    __device__ __forceinline__ f3 local(uint32_t r) const { return load3(pts, r); }
+    __device__ __forceinline__ float grad(uint32_t r, int ch) const {
+        return up[(int64_t)r * C + ch];
+    }
+};
+
+template <int C, class Src>
 __global__ void __launch_bounds__(256) det_app_reduce_kernel(
-    const uint32_t *__restrict__ key, const uint32_t *__restrict__ idx,
-    const float *__restrict__ aval, const uint32_t *__restrict__ bounds, int cls, int cbits,
-    int R, float *__restrict__ gdata) {
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ idx, Src src,
+    const uint32_t *__restrict__ bounds, int cls, int cbits, int R, float *__restrict__ gdata) {
     static_assert(C <= kWave, "one lane per channel");
-    constexpr int AP = det_app_stride(C);
     constexpr int NG = C <= kWave / 2 ? 2 : 1;  // runs in flight per wave
     constexpr int GW = kWave / NG;
     const int64_t lo = bounds[cls], hi = bounds[cls + 1];
@@ -282,11 +320,10 @@ __global__ void __launch_bounds__(256) det_app_reduce_kernel(
                 float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 2
                 for (int64_t i = start; i < end; ++i) {
-                    const float *rec = aval + (int64_t)idx[i] * AP;
-                    const float4 lp = *reinterpret_cast<const float4 *>(rec);
-                    const float g = ch < C ? rec[4 + ch] : 0.0f;
+                    const uint32_t r = idx[i];
+                    const float g = ch < C ? src.grad(r, ch) : 0.0f;
                     float w[8];
-                    trilinear_weights(f3{lp.x, lp.y, lp.z}, w);
+                    trilinear_weights(src.local(r), w);
 #pragma unroll
                     for (int v = 0; v < 8; ++v) acc[v] += w[v] * g;
                 }
@@ -314,8 +351,8 @@ int det_reduce(DetBuffers &b, int64_t T, float *gq, float *gdata, hipStream_t s)
     tb = b.temp_bytes;
     QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.akey, b.akey_s, b.aidx, b.aidx_s, (size_t)T,
                                         0u, (unsigned)(b.cbits + 4), s));
-    det_quadric_reduce_kernel<<<grid_1d(2 * T, 256), 256, 0, s>>>(b.qkey_s, b.qidx_s, b.qval,
-                                                                 2 * T, (uint32_t)R3, gq);
+    det_quadric_reduce_kernel<<<grid_1d(2 * T, 256), 256, 0, s>>>(
+        b.qkey_s, b.qidx_s, QuadricRecords{b.qval}, 2 * T, (uint32_t)R3, gq);
     QSDVR_LAUNCHED();
     det_class_bounds_kernel<<<1, 64, 0, s>>>(b.akey_s, T, b.cbits, b.bounds);
     QSDVR_LAUNCHED();
@@ -324,14 +361,211 @@ int det_reduce(DetBuffers &b, int64_t T, float *gq, float *gdata, hipStream_t s)
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 4096));
     for (int cls = 0; cls < 8; ++cls) {
         if (b.C == 32)
-            det_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(b.akey_s, b.aidx_s, b.aval, b.bounds,
-                                                             cls, b.cbits, b.R, gdata);
+            det_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(b.akey_s, b.aidx_s,
+                                                             AppRecords<32>{b.aval}, b.bounds, cls,
+                                                             b.cbits, b.R, gdata);
         else
-            det_app_reduce_kernel<53><<<blocks, 256, 0, s>>>(b.akey_s, b.aidx_s, b.aval, b.bounds,
-                                                             cls, b.cbits, b.R, gdata);
+            det_app_reduce_kernel<53><<<blocks, 256, 0, s>>>(b.akey_s, b.aidx_s,
+                                                             AppRecords<53>{b.aval}, b.bounds, cls,
+                                                             b.cbits, b.R, gdata);
         QSDVR_LAUNCHED();
     }
     return QSDVR_OK;
 }
 
+// ------------------------------------- op-surface sorted scatters ----
+// GridInterpolationBackward, SampleSDFBackward and SampleNormalBackward as
+// sorted-segment sums: the ops' own point lists, indices and upstream
+// gradients are the records; only the sample indices are sorted by key.
+
+// key of sample i: (parity class << cbits) | cell, invalid 8 << cbits
+__global__ void __launch_bounds__(256) op_interp_keys_kernel(const int32_t *__restrict__ idx,
+                                                             int64_t n, int R, int cbits,
+                                                             uint32_t *__restrict__ key,
+                                                             uint32_t *__restrict__ val) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t cell = idx[i];
+    uint32_t k = 8u << cbits;
+    if (cell >= 0 && cell < (int64_t)R * R * R) {
+        const int64_t x = cell % R, y = (cell / R) % R, z = cell / ((int64_t)R * R);
+        const uint32_t cls = (uint32_t)((x & 1) | ((y & 1) << 1) | ((z & 1) << 2));
+        k = (cls << cbits) | (uint32_t)cell;
+    }
+    key[i] = k;
+    val[i] = (uint32_t)i;
+}
+
+// key of sample i: its cell, invalid `cells`
+__global__ void __launch_bounds__(256) op_cell_keys_kernel(const int32_t *__restrict__ idx,
+                                                           int64_t n, int64_t cells,
+                                                           uint32_t *__restrict__ key,
+                                                           uint32_t *__restrict__ val) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t cell = idx[i];
+    key[i] = (cell >= 0 && cell < cells) ? (uint32_t)cell : (uint32_t)cells;
+    val[i] = (uint32_t)i;
+}
+
+namespace {
+
+struct OpBuffers {
+    uint32_t *key, *key_s, *val, *val_s, *bounds;
+    void *temp;
+    size_t temp_bytes;
+};
+
+// smallest b with 2^b >= n (cell bits of the interpolation keys)
+int bits_at_least(uint64_t n) {
+    int b = 1;
+    while ((1ull << b) < n) ++b;
+    return b;
+}
+
+size_t op_temp_bytes(int64_t n, int64_t cells) {
+    return std::max(sort_temp_bytes(n, bits_at_least((uint64_t)cells) + 4),
+                    sort_temp_bytes(n, bits_above((uint64_t)cells)));
+}
+
+size_t op_carve(void *ws, size_t bytes, int64_t n, int64_t cells, OpBuffers &b) {
+    Carver c{static_cast<char *>(ws), bytes};
+    b.key = c.take<uint32_t>(n);
+    b.key_s = c.take<uint32_t>(n);
+    b.val = c.take<uint32_t>(n);
+    b.val_s = c.take<uint32_t>(n);
+    b.bounds = c.take<uint32_t>(16);
+    b.temp_bytes = op_temp_bytes(n, cells);
+    b.temp = c.take<char>(b.temp_bytes);
+    return c.used + 256;
+}
+
+int op_prepare(void *ws, size_t bytes, int64_t n, int64_t cells, OpBuffers &b, const char *who) {
+    if (n >= 0xFFFFFFFFll || cells < 1 || cells >= 0x7FFFFFFFll) {
+        set_error("%s: %lld samples / %lld cells exceed the 32-bit sort keys", who, (long long)n,
+                  (long long)cells);
+        return QSDVR_ERANGE;
+    }
+    const size_t need = op_carve(nullptr, 0, n, cells, b);
+    if (ws == nullptr || bytes < need) {
+        set_error("%s: workspace of %zu bytes is too small (%zu needed)", who, bytes, need);
+        return QSDVR_EWORKSPACE;
+    }
+    op_carve(ws, bytes, n, cells, b);
+    return QSDVR_OK;
+}
+
+}  // namespace
+
 }  // namespace qsdvr
+
+using namespace qsdvr;
+
+extern "C" {
+
+size_t qsdvr_sorted_scatter_workspace_size(int64_t n, int64_t cell_count) {
+    if (n < 0 || cell_count < 1 || n >= 0xFFFFFFFFll || cell_count >= 0x7FFFFFFFll) return 0;
+    OpBuffers b;
+    return op_carve(nullptr, 0, n, cell_count, b);
+}
+
+int qsdvr_grid_interpolation_backward_sorted(float *grad_grid, const float *point,
+                                             const int32_t *index, const float *grad_out,
+                                             int64_t n, int32_t vertex_reso, int32_t channels,
+                                             void *workspace, size_t workspace_bytes,
+                                             qsdvr_stream stream) {
+    QSDVR_REQUIRE(n >= 0 && vertex_reso >= 3 && (vertex_reso - 1) % 2 == 0 &&
+                      (channels == 32 || channels == 53),
+                  "qsdvr_grid_interpolation_backward_sorted: bad sizes (even cell reso, "
+                  "channels 32 or 53)");
+    if (n == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(grad_grid && point && index && grad_out,
+                  "qsdvr_grid_interpolation_backward_sorted: null pointer");
+    const int R = vertex_reso - 1;
+    const int64_t cells = (int64_t)R * R * R;
+    QSDVR_REQUIRE(cells < (1ll << 27), "qsdvr_grid_interpolation_backward_sorted: reso too large");
+    OpBuffers b;
+    int st = op_prepare(workspace, workspace_bytes, n, cells,
+                        b, "qsdvr_grid_interpolation_backward_sorted");
+    if (st != QSDVR_OK) return st;
+    hipStream_t s = as_stream(stream);
+    const int cbits = bits_at_least((uint64_t)cells);
+    op_interp_keys_kernel<<<grid_1d(n, 256), 256, 0, s>>>(index, n, R, cbits, b.key, b.val);
+    QSDVR_LAUNCHED();
+    size_t tb = b.temp_bytes;
+    QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.key, b.key_s, b.val, b.val_s, (size_t)n, 0u,
+                                        (unsigned)(cbits + 4), s));
+    det_class_bounds_kernel<<<1, 64, 0, s>>>(b.key_s, n, cbits, b.bounds);
+    QSDVR_LAUNCHED();
+    const int64_t chunks = (n / 8 + kWave - 1) / kWave;
+    const unsigned blocks =
+        (unsigned)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 4096));
+    for (int cls = 0; cls < 8; ++cls) {
+        if (channels == 32)
+            det_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(
+                b.key_s, b.val_s, InterpSamples<32>{point, grad_out}, b.bounds, cls, cbits, R,
+                grad_grid);
+        else
+            det_app_reduce_kernel<53><<<blocks, 256, 0, s>>>(
+                b.key_s, b.val_s, InterpSamples<53>{point, grad_out}, b.bounds, cls, cbits, R,
+                grad_grid);
+        QSDVR_LAUNCHED();
+    }
+    return QSDVR_OK;
+}
+
+static int quadric_sorted(const int32_t *index, int64_t n, int64_t cells, void *workspace,
+                          size_t workspace_bytes, const char *who, hipStream_t s, OpBuffers &b) {
+    int st = op_prepare(workspace, workspace_bytes, n, cells, b, who);
+    if (st != QSDVR_OK) return st;
+    op_cell_keys_kernel<<<grid_1d(n, 256), 256, 0, s>>>(index, n, cells, b.key, b.val);
+    QSDVR_LAUNCHED();
+    size_t tb = b.temp_bytes;
+    QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.key, b.key_s, b.val, b.val_s, (size_t)n, 0u,
+                                        (unsigned)bits_above((uint64_t)cells), s));
+    return QSDVR_OK;
+}
+
+int qsdvr_sample_sdf_backward_sorted(const float *grad_out, const float *sdf_grid,
+                                     const float *point, const int32_t *index, float *grad_grid,
+                                     int64_t n, int64_t cell_count, float reso, void *workspace,
+                                     size_t workspace_bytes, qsdvr_stream stream) {
+    QSDVR_REQUIRE(n >= 0 && cell_count > 0 && reso != 0.0f,
+                  "qsdvr_sample_sdf_backward_sorted: bad sizes");
+    if (n == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(grad_out && sdf_grid && point && index && grad_grid,
+                  "qsdvr_sample_sdf_backward_sorted: null pointer");
+    hipStream_t s = as_stream(stream);
+    OpBuffers b;
+    int st = quadric_sorted(index, n, cell_count, workspace, workspace_bytes,
+                            "qsdvr_sample_sdf_backward_sorted", s, b);
+    if (st != QSDVR_OK) return st;
+    det_quadric_reduce_kernel<<<grid_1d(n, 256), 256, 0, s>>>(
+        b.key_s, b.val_s, SdfSamples{sdf_grid, point, grad_out, 1.0f / reso}, n,
+        (uint32_t)cell_count, grad_grid);
+    QSDVR_LAUNCHED();
+    return QSDVR_OK;
+}
+
+int qsdvr_sample_normal_backward_sorted(const float *grad_out, const float *sdf_grid,
+                                        const float *point, const int32_t *index,
+                                        float *grad_grid, int64_t n, int64_t cell_count,
+                                        void *workspace, size_t workspace_bytes,
+                                        qsdvr_stream stream) {
+    QSDVR_REQUIRE(n >= 0 && cell_count > 0, "qsdvr_sample_normal_backward_sorted: bad sizes");
+    if (n == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(grad_out && sdf_grid && point && index && grad_grid,
+                  "qsdvr_sample_normal_backward_sorted: null pointer");
+    hipStream_t s = as_stream(stream);
+    OpBuffers b;
+    int st = quadric_sorted(index, n, cell_count, workspace, workspace_bytes,
+                            "qsdvr_sample_normal_backward_sorted", s, b);
+    if (st != QSDVR_OK) return st;
+    det_quadric_reduce_kernel<<<grid_1d(n, 256), 256, 0, s>>>(
+        b.key_s, b.val_s, NormalSamples{sdf_grid, point, grad_out}, n, (uint32_t)cell_count,
+        grad_grid);
+    QSDVR_LAUNCHED();
+    return QSDVR_OK;
+}
+
+}  // extern "C"

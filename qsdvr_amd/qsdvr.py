@@ -159,6 +159,12 @@ def GridInterpolationBackward(dataGrid, pointList, indexList, out, reso):
     R1 = int(reso)
     if dataGrid.numel() != R1 ** 3 * Cc:
         raise ValueError(f"dataGrid has {dataGrid.numel()} elements, expected ({R1})^3 x {Cc}")
+    if SORTED_SCATTER and Cc in (32, 53) and (R1 - 1) % 2 == 0 and n > 0:
+        ws_n = _sorted_workspace(n, (R1 - 1) ** 3)
+        ws = _workspace(ws_n)
+        _lib.call("qsdvr_grid_interpolation_backward_sorted", _ptr(dataGrid), _ptr(pointList),
+                  _ptr(indexList), _ptr(out), n, R1, Cc, _ptr(ws), ws_n, _stream())
+        return
     _lib.call("qsdvr_grid_interpolation_backward", _ptr(dataGrid), _ptr(pointList),
               _ptr(indexList), _ptr(out), n, R1, Cc, _stream())
 
@@ -254,6 +260,13 @@ def SampleSDFBackward(out, sdfGrid, pointList, indexList, sdfGradGrid, reso):
         raise ValueError("sdfGradGrid must match sdfGrid")
     if float(reso) == 0.0:
         raise ValueError("reso must be nonzero")
+    if SORTED_SCATTER and n > 0:
+        ws_n = _sorted_workspace(n, cells)
+        ws = _workspace(ws_n)
+        _lib.call("qsdvr_sample_sdf_backward_sorted", _ptr(out), _ptr(sdfGrid), _ptr(pointList),
+                  _ptr(indexList), _ptr(sdfGradGrid), n, cells, float(reso), _ptr(ws), ws_n,
+                  _stream())
+        return
     _lib.call("qsdvr_sample_sdf_backward", _ptr(out), _ptr(sdfGrid), _ptr(pointList),
               _ptr(indexList), _ptr(sdfGradGrid), n, cells, float(reso), _stream())
 
@@ -277,6 +290,13 @@ def SampleNormalBackward(out, sdfGrid, pointList, indexList, sdfGradGrid):
     _t("sdfGradGrid", sdfGradGrid, torch.float32)
     if sdfGradGrid.numel() != sdfGrid.numel():
         raise ValueError("sdfGradGrid must match sdfGrid")
+    if SORTED_SCATTER and n > 0:
+        ws_n = _sorted_workspace(n, cells)
+        ws = _workspace(ws_n)
+        _lib.call("qsdvr_sample_normal_backward_sorted", _ptr(out), _ptr(sdfGrid),
+                  _ptr(pointList), _ptr(indexList), _ptr(sdfGradGrid), n, cells, _ptr(ws), ws_n,
+                  _stream())
+        return
     _lib.call("qsdvr_sample_normal_backward", _ptr(out), _ptr(sdfGrid), _ptr(pointList),
               _ptr(indexList), _ptr(sdfGradGrid), n, cells, _stream())
 
@@ -295,6 +315,22 @@ def _layers(x, y, z, off):
 
 def _workspace(nbytes):
     return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device="cuda")
+
+
+# The three accumulating backward ops (GridInterpolationBackward,
+# SampleSDFBackward, SampleNormalBackward) run as sorted-segment sums: the
+# same result as one float atomic per sample and coefficient, bitwise
+# reproducible and several times faster on per-pose sample lists, where the
+# samples of many rays pile onto the same cells.  False selects the atomic
+# kernels.
+SORTED_SCATTER = True
+
+
+def _sorted_workspace(n, cells):
+    nb = int(_lib.fn("qsdvr_sorted_scatter_workspace_size")(int(n), int(cells)))
+    if nb == 0:
+        raise ValueError(f"{n} samples / {cells} cells exceed the sorted scatter's 32-bit keys")
+    return nb
 
 
 def LayerToGridForward(out, xLayer, yLayer, zLayer, offset):

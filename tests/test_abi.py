@@ -88,3 +88,23 @@ def test_deterministic_backward_validates_on_the_host():
                                       dummy, None, dummy, None, dummy, dummy, 100, dummy, 1 << 30,
                                       None)
     assert st == -1 and b"512" in lib.qsdvr_last_error()
+
+
+def test_sorted_scatters_validate_on_the_host():
+    lib = _lib.load()
+    nb = lib.qsdvr_sorted_scatter_workspace_size(1000, 64 ** 3)
+    assert nb >= 4 * 4 * 1000
+    assert lib.qsdvr_sorted_scatter_workspace_size(1000, 0) == 0
+    dummy = C.c_void_p(16)
+    # odd cell resolution and unsupported channel counts are refused
+    st = lib.qsdvr_grid_interpolation_backward_sorted(dummy, dummy, dummy, dummy, 100, 10, 32,
+                                                      dummy, 1 << 20, None)
+    assert st == -1 and b"even" in lib.qsdvr_last_error()
+    st = lib.qsdvr_grid_interpolation_backward_sorted(dummy, dummy, dummy, dummy, 100, 9, 31,
+                                                      dummy, 1 << 20, None)
+    assert st == -1
+    st = lib.qsdvr_sample_sdf_backward_sorted(dummy, dummy, dummy, dummy, dummy, 100, 512, 8.0,
+                                              dummy, 16, None)
+    assert st == -4 and b"too small" in lib.qsdvr_last_error()
+    assert lib.qsdvr_sample_normal_backward_sorted(None, None, None, None, None, 0, 512, None, 0,
+                                                   None) == 0
