@@ -106,84 +106,133 @@ __global__ void __launch_bounds__(256) shader_bwd_kernel(const float *__restrict
 }
 
 // -------------------------------------------------------- RayAggregate ----
-// NeuS logistic compositing (render_torch.py:155-167), one lane per ray.
+// NeuS logistic compositing (render_torch.py:155-167).
 __device__ __forceinline__ bool ray_ok(int4 r, int64_t sdf_total, int64_t render_total) {
     uint32_t sc = (uint32_t)r.x, so = (uint32_t)r.y, rc = (uint32_t)r.z, ro = (uint32_t)r.w;
     if (rc == 0) return true;
     return (int64_t)so + rc + 1 <= sdf_total && (int64_t)ro + rc <= render_total && sc >= rc;
 }
 
+// One wavefront per ray, lanes over 64 consecutive samples at a time (the
+// reference walks each ray serially in one thread, kernel/render.cu:223-253).
+// T_i is the carried transmittance times an exclusive product scan of
+// (1 - alpha) over the lanes; the colour is each lane's sum, reduced over
+// the wave in a fixed order at the end.
 __global__ void __launch_bounds__(256) aggregate_fwd_kernel(
     float *__restrict__ out, const float *__restrict__ rgb, const float *__restrict__ sdf,
     const int32_t *__restrict__ ray, float *__restrict__ alpha, float *__restrict__ T_out,
     int64_t rays, int64_t sdf_total, int64_t render_total, float s) {
-    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= rays) return;
-    int4 r = reinterpret_cast<const int4 *>(ray)[id];
-    f3 col{0.0f, 0.0f, 0.0f};
+    const int lane = threadIdx.x % kWave;
+    const int64_t id = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (id >= rays) return;  // whole wavefront
+    const int4 r = reinterpret_cast<const int4 *>(ray)[id];
     if (!ray_ok(r, sdf_total, render_total)) {
-        float q = __builtin_nanf("");
-        store3(out, id, f3{q, q, q});
+        const float q = __builtin_nanf("");
+        if (lane == 0) store3(out, id, f3{q, q, q});
         return;
     }
-    uint32_t sc = (uint32_t)r.x, so = (uint32_t)r.y, rc = (uint32_t)r.z, ro = (uint32_t)r.w;
+    const uint32_t sc = (uint32_t)r.x, so = (uint32_t)r.y, rc = (uint32_t)r.z, ro = (uint32_t)r.w;
+    f3 col{0.0f, 0.0f, 0.0f};
     if (sc != 0 && rc != 0) {
-        float e0 = expf(-s * sdf[so]);
         float T = 1.0f;
-        for (uint32_t i = 0; i < rc; ++i) {
-            float e1 = expf(-s * sdf[(int64_t)so + i + 1]);
-            float a = fminf(fmaxf((e1 - e0) / (1.0f + e1), 0.0f), 10.0f);
-            alpha[(int64_t)ro + i] = a;
-            T_out[(int64_t)ro + i] = T;
-            f3 c = load3(rgb, (int64_t)ro + i);
-            col.x += c.x * a * T;
-            col.y += c.y * a * T;
-            col.z += c.z * a * T;
-            e0 = e1;
-            T = T * (1.0f - a);
+        for (uint32_t c0 = 0; c0 < rc; c0 += kWave) {
+            const uint32_t i = c0 + lane;
+            const bool valid = i < rc;
+            const float e0 = i <= rc ? expf(-s * sdf[(int64_t)so + i]) : 0.0f;
+            float e1 = __shfl_down(e0, 1, kWave);
+            if (lane == kWave - 1 && i + 1 <= rc) e1 = expf(-s * sdf[(int64_t)so + i + 1]);
+            const float a = valid ? fminf(fmaxf((e1 - e0) / (1.0f + e1), 0.0f), 10.0f) : 0.0f;
+            float p = 1.0f - a;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const float u = __shfl_up(p, o, kWave);
+                if (lane >= o) p *= u;
+            }
+            const float up = __shfl_up(p, 1, kWave);
+            const float Ti = T * (lane == 0 ? 1.0f : up);
+            if (valid) {
+                alpha[(int64_t)ro + i] = a;
+                T_out[(int64_t)ro + i] = Ti;
+                const f3 c = load3(rgb, (int64_t)ro + i);
+                col.x += c.x * a * Ti;
+                col.y += c.y * a * Ti;
+                col.z += c.z * a * Ti;
+            }
+            T *= __shfl(p, kWave - 1, kWave);
         }
     }
-    store3(out, id, col);
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        col.x += __shfl_xor(col.x, o, kWave);
+        col.y += __shfl_xor(col.y, o, kWave);
+        col.z += __shfl_xor(col.z, o, kWave);
+    }
+    if (lane == 0) store3(out, id, col);
 }
 
 // Adjoint with the saved alpha and T.  dL/dalpha_k = T_k (g.rgb_k - rho_k)
-// where rho is the suffix colour seen after sample k, built back to front by
-// rho_{k-1} = alpha_k (g.rgb_k) + (1 - alpha_k) rho_k -- no division by
-// (1 - alpha).  The clamp passes the gradient wherever the pre-clamp value
-// is >= 0 (torch.clamp semantics).
+// where rho is the suffix colour seen after sample k, rho_{k-1} =
+// alpha_k (g.rgb_k) + (1 - alpha_k) rho_k -- no division by (1 - alpha).
+// One wavefront per ray, chunks of 64 samples from the back: rho over a
+// chunk is a reverse scan of the affine maps x -> (1 - alpha_k) x +
+// alpha_k g.rgb_k applied to the carried rho of the later chunks.  The
+// clamp passes the gradient wherever the pre-clamp value is in [0, 10]
+// (torch.clamp semantics).
 __global__ void __launch_bounds__(256) aggregate_bwd_kernel(
     const float *__restrict__ up, const float *__restrict__ rgb, const float *__restrict__ sdf,
     const int32_t *__restrict__ ray, const float *__restrict__ alpha,
     const float *__restrict__ T_in, float *__restrict__ grgb, float *__restrict__ gsdf,
     int64_t rays, int64_t sdf_total, int64_t render_total, float s) {
-    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= rays) return;
-    int4 r = reinterpret_cast<const int4 *>(ray)[id];
-    uint32_t sc = (uint32_t)r.x, so = (uint32_t)r.y, rc = (uint32_t)r.z, ro = (uint32_t)r.w;
+    const int lane = threadIdx.x % kWave;
+    const int64_t id = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (id >= rays) return;  // whole wavefront
+    const int4 r = reinterpret_cast<const int4 *>(ray)[id];
+    const uint32_t sc = (uint32_t)r.x, so = (uint32_t)r.y, rc = (uint32_t)r.z, ro = (uint32_t)r.w;
     if (sc == 0 || rc == 0 || !ray_ok(r, sdf_total, render_total)) return;
-    f3 g = load3(up, id);
-    float rho = 0.0f;
-    float e1 = expf(-s * sdf[(int64_t)so + rc]);
-    float carry = 0.0f;  // contribution to gsdf[so + k + 1] from alpha_{k+1}
-    for (int64_t k = (int64_t)rc - 1; k >= 0; --k) {
-        float e0 = expf(-s * sdf[so + k]);
-        float a = alpha[ro + k], T = T_in[ro + k];
-        f3 c = load3(rgb, ro + k);
-        float gc = g.x * c.x + g.y * c.y + g.z * c.z;
-        float da = T * (gc - rho);
-        float inv = 1.0f / (1.0f + e1);
-        float pre = (e1 - e0) * inv;
-        float pass = (pre >= 0.0f && pre <= 10.0f) ? da : 0.0f;
-        float d_s0 = pass * s * e0 * inv;                        // d alpha / d sdf_k
-        float d_s1 = -pass * s * e1 * (1.0f + e0) * inv * inv;   // d alpha / d sdf_{k+1}
-        gsdf[so + k + 1] = carry + d_s1;
-        carry = d_s0;
-        float w = a * T;
-        store3(grgb, ro + k, f3{g.x * w, g.y * w, g.z * w});
-        rho = a * gc + (1.0f - a) * rho;
-        e1 = e0;
+    const f3 g = load3(up, id);
+    float rho_after = 0.0f;  // rho at the last sample of the chunk
+    float ds0_next = 0.0f;   // d alpha / d sdf of the first sample of the later chunk
+    for (int64_t c0 = ((int64_t)rc - 1) / kWave * kWave; c0 >= 0; c0 -= kWave) {
+        const int64_t k = c0 + lane;
+        const bool valid = k < (int64_t)rc;
+        float a = 0.0f, T = 0.0f, gc = 0.0f, e0 = 0.0f, e1 = 0.0f;
+        if (valid) {
+            a = alpha[(int64_t)ro + k];
+            T = T_in[(int64_t)ro + k];
+            const f3 c = load3(rgb, (int64_t)ro + k);
+            gc = g.x * c.x + g.y * c.y + g.z * c.z;
+            e0 = expf(-s * sdf[(int64_t)so + k]);
+            e1 = expf(-s * sdf[(int64_t)so + k + 1]);
+        }
+        // inclusive suffix composition of the maps of lanes l..63
+        float A = valid ? 1.0f - a : 1.0f, B = valid ? a * gc : 0.0f;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const float A2 = __shfl_down(A, o, kWave), B2 = __shfl_down(B, o, kWave);
+            if (lane + o < kWave) {
+                B = A * B2 + B;
+                A = A * A2;
+            }
+        }
+        const float An = __shfl_down(A, 1, kWave), Bn = __shfl_down(B, 1, kWave);
+        const float rho = lane == kWave - 1 ? rho_after : An * rho_after + Bn;
+        const float da = T * (gc - rho);
+        const float inv = 1.0f / (1.0f + e1);
+        const float pre = (e1 - e0) * inv;
+        const float pass = (valid && pre >= 0.0f && pre <= 10.0f) ? da : 0.0f;
+        const float d_s0 = pass * s * e0 * inv;                        // d alpha_k / d sdf_k
+        const float d_s1 = -pass * s * e1 * (1.0f + e0) * inv * inv;   // d alpha_k / d sdf_{k+1}
+        float nxt = __shfl_down(d_s0, 1, kWave);
+        if (lane == kWave - 1) nxt = ds0_next;
+        if (valid) {
+            gsdf[(int64_t)so + k + 1] = nxt + d_s1;
+            const float w = a * T;
+            store3(grgb, (int64_t)ro + k, f3{g.x * w, g.y * w, g.z * w});
+        }
+        ds0_next = __shfl(d_s0, 0, kWave);
+        rho_after = __shfl(A * rho_after + B, 0, kWave);
     }
-    gsdf[so] = carry;
+    if (lane == 0) gsdf[so] = ds0_next;
 }
 
 }  // namespace qsdvr
@@ -283,7 +332,7 @@ int qsdvr_ray_aggregate_forward(float *out, const float *rgb, const float *sdf,
     QSDVR_REQUIRE(out && ray_list, "qsdvr_ray_aggregate_forward: null pointer");
     QSDVR_REQUIRE(render_total == 0 || (rgb && sdf && alpha && T),
                   "qsdvr_ray_aggregate_forward: null sample list");
-    aggregate_fwd_kernel<<<grid_1d(ray_count, 256), 256, 0, as_stream(stream)>>>(
+    aggregate_fwd_kernel<<<grid_1d(ray_count, 4), 256, 0, as_stream(stream)>>>(
         out, rgb, sdf, ray_list, alpha, T, ray_count, sdf_total, render_total, logistic_coef);
     QSDVR_LAUNCHED();
     return QSDVR_OK;
@@ -300,7 +349,7 @@ int qsdvr_ray_aggregate_backward(const float *grad_out, const float *rgb, const 
     QSDVR_REQUIRE(grad_out && ray_list, "qsdvr_ray_aggregate_backward: null pointer");
     QSDVR_REQUIRE(render_total == 0 || (rgb && sdf && alpha && T && rgb_grad && sdf_grad),
                   "qsdvr_ray_aggregate_backward: null sample list");
-    aggregate_bwd_kernel<<<grid_1d(ray_count, 256), 256, 0, as_stream(stream)>>>(
+    aggregate_bwd_kernel<<<grid_1d(ray_count, 4), 256, 0, as_stream(stream)>>>(
         grad_out, rgb, sdf, ray_list, alpha, T, rgb_grad, sdf_grad, ray_count, sdf_total,
         render_total, logistic_coef);
     QSDVR_LAUNCHED();
