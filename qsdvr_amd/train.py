@@ -55,6 +55,23 @@ class DeviceAdam:
         for p in self.params:
             p.grad = None
 
+    def state_dict(self):
+        """Moments and step count (host copies), for checkpoints."""
+        return {"t": self.t, "lr": self.lr, "betas": (self.b1, self.b2), "eps": self.eps,
+                "m": [m.detach().cpu() for m in self.m], "v": [v.detach().cpu() for v in self.v]}
+
+    def load_state_dict(self, sd):
+        if len(sd["m"]) != len(self.m):
+            raise ValueError("optimiser state has a different parameter list")
+        self.t = int(sd["t"])
+        self.lr = float(sd["lr"])
+        self.b1, self.b2 = map(float, sd["betas"])
+        self.eps = float(sd["eps"])
+        for dst, src in zip(self.m + self.v, list(sd["m"]) + list(sd["v"])):
+            if dst.shape != src.shape:
+                raise ValueError("optimiser state shape mismatch")
+            dst.copy_(src)
+
 
 class FusedTrainer:
     """One training step = one random batch of rays over ``n_cams`` cameras
@@ -91,6 +108,31 @@ class FusedTrainer:
         n_all = self.n_cams * self.W * self.H
         return torch.randint(0, n_all, (batch,), generator=self.gen, device=self.data.device,
                              dtype=torch.int64).to(torch.int32)
+
+    def state_dict(self):
+        """Parameters, optimiser state and the batch generator (host
+        copies): ``torch.save`` it, and ``load_state_dict`` resumes the run
+        where it stopped."""
+        params = {"data": self.data.detach().cpu()}
+        if self.sdf is not None:
+            params["sdf"] = self.sdf.detach().cpu()
+        else:
+            params["layers"] = [t.detach().cpu() for t in self.layers]
+        return {"params": params, "adam": self.opt.state_dict(),
+                "gen": self.gen.get_state(), "s": self.s}
+
+    @torch.no_grad()
+    def load_state_dict(self, sd):
+        p = sd["params"]
+        self.data.copy_(p["data"])
+        if self.sdf is not None:
+            self.sdf.copy_(p["sdf"])
+        else:
+            for dst, src in zip(self.layers, p["layers"]):
+                dst.copy_(src)
+        self.opt.load_state_dict(sd["adam"])
+        self.gen.set_state(sd["gen"])
+        self.s = float(sd["s"])
 
     def step(self, rays, target):
         """Forward + MSE + adjoint + Adam on ``rays`` (int32 codes) against

@@ -96,3 +96,33 @@ def test_deterministic_training_repeats_bitwise():
         finals.append((tr.sdf.detach().clone(), tr.data.detach().clone()))
     assert torch.equal(finals[0][0], finals[1][0])
     assert torch.equal(finals[0][1], finals[1][1])
+
+
+def test_checkpoint_resume_is_bitwise(tmp_path):
+    """Three steps, checkpoint, three more steps == resume from the file and
+    run the same three steps (deterministic scatter)."""
+    w, grid, data, cams, cam_buf = _small_scene()
+    target = torch.rand((len(cams) * w.width * w.height, 3),
+                        generator=torch.Generator(device="cuda").manual_seed(6), device="cuda")
+
+    def make():
+        return FusedTrainer(data.clone(), cam_buf, len(cams), w.width, w.height, w.interval, w.s,
+                            sdf_grid=grid.clone(), lr=1e-2, seed=9)
+
+    a = make()
+    for _ in range(3):
+        r = a.draw(2048)
+        a.step(r, target[r.long()])
+    path = tmp_path / "trainer.pt"
+    torch.save(a.state_dict(), path)
+    for _ in range(3):
+        r = a.draw(2048)
+        a.step(r, target[r.long()])
+    b = make()
+    b.load_state_dict(torch.load(path, weights_only=True))
+    for _ in range(3):
+        r = b.draw(2048)
+        b.step(r, target[r.long()])
+    torch.cuda.synchronize()
+    assert torch.equal(a.sdf, b.sdf) and torch.equal(a.data, b.data)
+    assert a.opt.t == b.opt.t == 6
