@@ -150,6 +150,91 @@ __device__ __forceinline__ void clamp_cell(CellPoint &cp, int R) {
     }
 }
 
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Orders this wave's LDS accesses for the compiler only.  A wave's LDS
+// instructions execute in issue order, so LDS data written by one lane is
+// visible to a later read by another lane of the same wave without
+// s_barrier (each wave below works in its own LDS region).
+__device__ __forceinline__ void wave_lds_order() { __asm__ volatile("" ::: "memory"); }
+
+// Appearance staging box: the vertex rows of the box of cells that a wave's
+// samples fall in at one step, copied to LDS once by the whole wave (64
+// lanes x 16 B per load) and read there by each lane, in place of 64
+// per-lane 16-B gathers per sample.  kBoxV vertices of C = 32 channels per
+// wave (8 KB); the quads of vertex v are XOR-swizzled by v's low bits, so
+// lanes reading different vertices at one quad index hit different banks.
+constexpr int kBoxV = 64;
+
+struct BoxView {
+    const float4 *lds;  // nullptr: gather from global memory
+    int vb, ex, sxy;    // the lane's cell-origin vertex in the box, box strides
+};
+
+template <int Q>
+__device__ __forceinline__ int box_slot(int v, int q) {
+    static_assert((Q & (Q - 1)) == 0, "quads per row must be a power of two");
+    return v * Q + (q ^ (v & (Q - 1)));
+}
+
+// gather_interp from the staged box (same operation order).
+template <int C>
+__device__ __forceinline__ void box_interp(const BoxView &bx, const float w[8], float d[C]) {
+    constexpr int Q = C / 4;
+#pragma unroll
+    for (int c = 0; c < C; ++c) d[c] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int v = bx.vb + (k & 1) + ((k >> 1) & 1) * bx.ex + (k >> 2) * bx.sxy;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const float4 val = bx.lds[box_slot<Q>(v, q)];
+            d[4 * q + 0] += w[k] * val.x;
+            d[4 * q + 1] += w[k] * val.y;
+            d[4 * q + 2] += w[k] * val.z;
+            d[4 * q + 3] += w[k] * val.w;
+        }
+    }
+}
+
+// Stage the box of the cells of the lanes with ``need`` set (their cell
+// ``rp``): returns the lane's view of it, or a null view (gather from global
+// memory) when the box has more than kBoxV vertices.  Called by the whole
+// wave (wave-uniform control flow).
+template <int C>
+__device__ __forceinline__ BoxView stage_box(float4 *box, const float *__restrict__ data, int R,
+                                             bool need, const CellPoint &rp, int lane) {
+    constexpr int Q = C / 4;
+    int lx = need ? rp.ix : INT_MAX, ly = need ? rp.iy : INT_MAX, lz = need ? rp.iz : INT_MAX;
+    int hx = need ? rp.ix : INT_MIN, hy = need ? rp.iy : INT_MIN, hz = need ? rp.iz : INT_MIN;
+    lx = wave_min(lx); ly = wave_min(ly); lz = wave_min(lz);
+    hx = wave_max(hx); hy = wave_max(hy); hz = wave_max(hz);
+    const int ex = hx - lx + 2, ey = hy - ly + 2, ez = hz - lz + 2;
+    const int sxy = ex * ey, nv = sxy * ez;
+    if (nv > kBoxV) return BoxView{nullptr, 0, 0, 0};
+    const int64_t RR = (int64_t)R * R;
+    const float4 *__restrict__ src = reinterpret_cast<const float4 *>(data);
+    wave_lds_order();  // the previous step's box reads are issued before these writes
+    for (int i = lane; i < nv * Q; i += kWave) {
+        const int v = i / Q, q = i % Q;
+        const int vz = v / sxy, r2 = v - vz * sxy, vy = r2 / ex, vx = r2 - vy * ex;
+        const int64_t row = (int64_t)(lx + vx) + (int64_t)(ly + vy) * R + (int64_t)(lz + vz) * RR;
+        box[box_slot<Q>(v, q)] = src[row * Q + q];
+    }
+    wave_lds_order();
+    return BoxView{box, (rp.ix - lx) + (rp.iy - ly) * ex + (rp.iz - lz) * sxy, ex, sxy};
+}
+
 struct SampleEval {
     CellPoint rp;     // render sample cell / local point
     f3 normal;
@@ -165,6 +250,50 @@ __device__ __forceinline__ CellPoint point_at(const RaySetup &ray, float t, floa
     return to_cell(qx, qy, qz, R);
 }
 
+// Render sample j of the ray (cell clamped as clamp_cell does).
+__device__ __forceinline__ CellPoint render_point(const RaySetup &ray, uint32_t j, float interval,
+                                                  float half_reso, int R) {
+    CellPoint rp;
+    if (ray.tl) {
+        const float sj = 0.5f * (ray.tl[j] + ray.tl[j + 1]);
+        rp = point_at(ray, sj, half_reso, R);
+    } else {
+        rp = render_sample(ray.ori, ray.dir, j, interval, half_reso, R);
+    }
+    clamp_cell(rp, R);
+    return rp;
+}
+
+// Render sample j at its cell point rp: normal, appearance (from the staged
+// box when bx.lds is set, else from global memory), shading and distance.
+template <int NC>
+__device__ __forceinline__ void eval_render_at(const RaySetup &ray, uint32_t j, float interval,
+                                               const CellPoint &rp, int R,
+                                               const float *__restrict__ qgrid,
+                                               const float *__restrict__ data, const BoxView &bx,
+                                               float w[8], float d[3 * NC + 5], SampleEval &e) {
+    constexpr int C = 3 * NC + 5;
+    const float sj = ray.tl ? 0.5f * (ray.tl[j] + ray.tl[j + 1]) : ((float)j + 0.5f) * interval;
+    e.rp = rp;
+    const Quadric q = load_quadric(qgrid, e.rp.cell);
+    e.normal = quadric_normal(q, e.rp.local);
+    trilinear_weights(e.rp.local, w);
+    if constexpr (C == 32) {
+        if (bx.lds != nullptr) box_interp<C>(bx, w, d);
+        else gather_interp<C>(data, e.rp.cell, R, w, d);
+    } else {
+        gather_interp<C>(data, e.rp.cell, R, w, d);
+    }
+    ShadeOut so = shade<NC>(e.normal, ray.dir, d);
+    e.rgb = so.rgb;
+    e.spec = so.spec;
+    const float px = ray.ori.x + ray.dir.x * sj - ray.center.x;
+    const float py = ray.ori.y + ray.dir.y * sj - ray.center.y;
+    const float pz = ray.ori.z + ray.dir.z * sj - ray.center.z;
+    e.t = sqrtf(px * px + py * py + pz * pz);
+}
+
+// Render sample j with per-lane gathers from global memory.
 template <int NC>
 __device__ __forceinline__ void eval_render_sample(const RaySetup &ray, uint32_t j, float interval,
                                                    float half_reso, int R,
@@ -207,36 +336,59 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
     const float4 *__restrict__ ray_state, const float *__restrict__ tlist, int K,
     float *__restrict__ bins) {
     constexpr int C = 3 * NC + 5;
+    extern __shared__ float4 fwd_lds[];
     RaySetup ray = setup_ray(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state, tlist, K);
-    if (ray.out < 0) return;
+    const bool live = ray.out >= 0 && ray.renders > 0;
+    // Tile waves (rays of one 8x8 pixel tile) stage their appearance rows in
+    // LDS; waves of listed rays gather per lane.  The step loop runs
+    // wave-uniformly (lanes past their last sample idle) for the staging.
+    const bool stage = C == 32 && ray_pixel == nullptr;
+    const int lane = threadIdx.x % kWave;
+    float4 *box = fwd_lds + (threadIdx.x / kWave) * (kBoxV * C / 4);
     const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
     f3 col{0.0f, 0.0f, 0.0f};
     float dep = 0.0f;
-    if (ray.renders > 0) {
-        CellPoint cp;
-        float e0 = expf(-s * sdf_at(ray, 0, interval, half_reso, R, inv_reso, qgrid, cp));
-        float T = 1.0f;
-        // 64 equal bins over the ray's sampled segment (training targets).
-        // The bin index never decreases along the ray, so each bin is summed
-        // in a register and stored once (the caller zeroes the bins).
-        int bin = 0;
-        float bin_sum = 0.0f;
-        for (uint32_t j = 0; j < ray.renders; ++j) {
-            float e1 = expf(-s * sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp));
-            float a = fmaxf((e1 - e0) / (1.0f + e1), 0.0f);
-            float wt = a * T;
-            // A sample of weight exactly zero (alpha clamped to 0 where the
-            // SDF rises along the ray, or T underflowed) adds nothing: its
-            // appearance gather and shading are skipped.
-            if (wt != 0.0f) {
+    const int steps = wave_max(live ? (int)ray.renders : 0);
+    CellPoint cp;
+    float e0 = live ? expf(-s * sdf_at(ray, 0, interval, half_reso, R, inv_reso, qgrid, cp)) : 1.0f;
+    float T = 1.0f;
+    // 64 equal bins over the ray's sampled segment (training targets).
+    // The bin index never decreases along the ray, so each bin is summed
+    // in a register and stored once (the caller zeroes the bins).
+    int bin = 0;
+    float bin_sum = 0.0f;
+    for (int j = 0; j < steps; ++j) {
+        const bool act = live && j < (int)ray.renders;
+        float a = 0.0f, wt = 0.0f, e1 = 0.0f;
+        if (act) {
+            e1 = expf(-s * sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp));
+            a = fmaxf((e1 - e0) / (1.0f + e1), 0.0f);
+            wt = a * T;
+        }
+        // A sample of weight exactly zero (alpha clamped to 0 where the SDF
+        // rises along the ray, or T underflowed) adds nothing: its
+        // appearance gather and shading are skipped.
+        const bool need = wt != 0.0f;
+        if (__ballot(need) != 0) {
+            CellPoint rp;
+            rp.ix = rp.iy = rp.iz = 0;
+            rp.cell = 0;
+            if (need) rp = render_point(ray, (uint32_t)j, interval, half_reso, R);
+            BoxView bx{nullptr, 0, 0, 0};
+            if constexpr (C == 32) {
+                if (stage) bx = stage_box<C>(box, data, R, need, rp, lane);
+            }
+            if (need) {
                 float w[8], d[C];
                 SampleEval ev;
-                eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
+                eval_render_at<NC>(ray, (uint32_t)j, interval, rp, R, qgrid, data, bx, w, d, ev);
                 col.x += ev.rgb.x * wt;
                 col.y += ev.rgb.y * wt;
                 col.z += ev.rgb.z * wt;
                 dep += ev.t * wt;
             }
+        }
+        if (act) {
             if constexpr (BINS) {
                 const int b = min((int)(((float)j + 0.5f) / (float)ray.renders * 64.0f), 63);
                 if (b != bin) {
@@ -249,7 +401,10 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
             T = T * (1.0f - a);
             e0 = e1;
         }
-        if constexpr (BINS) bins[ray.out * 64 + bin] = bin_sum;
+    }
+    if (ray.out < 0) return;
+    if constexpr (BINS) {
+        if (live) bins[ray.out * 64 + bin] = bin_sum;
     }
     store3(color, ray.out, col);
     if (depth) depth[ray.out] = dep;
@@ -519,18 +674,6 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
 constexpr int kCapV = 64;
 static_assert(kCapV >= 64, "a fallback round's 3x3x3-cell box has 64 vertices");
 
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
-    return v;
-}
-
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
-    return v;
-}
-
 // DPP lane shifts within rows of 16 lanes (row_shr:D = 0x110 + D, row_shl:1
 // = 0x101); a lane whose source lies outside its row reads 0
 template <int CTRL>
@@ -593,12 +736,6 @@ __device__ __forceinline__ int row_run_merge(float v[7], int key, int lane) {
     }
     return tail;
 }
-
-// Orders this wave's LDS accesses for the compiler only.  The tile kernels
-// run one wave per workgroup, and a wave's LDS instructions execute in
-// issue order, so LDS data written by one lane is visible to a later read
-// by another lane of the wave without s_barrier.
-__device__ __forceinline__ void wave_lds_order() { __asm__ volatile("" ::: "memory"); }
 
 // Quadric gradient box: a 4x4x4-cell window of the quadric gradient grid
 // held in LDS by one wave ([coefficient][cell], 7 x 64 floats).  The merged
@@ -822,7 +959,9 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
             float a = fmaxf(praw, 0.0f);
             const float wt = a * T;
             // the render sample is evaluated only where it adds to the prefix
-            // or to dL/dalpha; a zero-weight sample has zero gradients
+            // or to dL/dalpha; a zero-weight sample has zero gradients.  (Its
+            // vertex rows are gathered per lane: staging them in LDS as the
+            // forward does measured slower here, at 2 waves per SIMD.)
             if (praw >= 0.0f && T != 0.0f) {
                 float d[C];
                 SampleEval ev;
@@ -949,11 +1088,15 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                 const int g = ln >> 4, m = ln & 15;
                 const int k0 = (2 * g) ^ ((ln >> 1) & 7), k1 = k0 ^ 1;
                 const int nmb = (nv + 15) >> 4;
+                // NBG channel blocks share one staging of the weight columns
+                constexpr int NBG = CP / 16 >= 2 ? 2 : 1;
 #pragma unroll
-                for (int nb = 0; nb < CP / 16; ++nb) {
-                    floatx4 acc[4];
+                for (int nb0 = 0; nb0 < CP / 16; nb0 += NBG) {
+                    floatx4 acc[NBG][4];
 #pragma unroll
-                    for (int mb = 0; mb < 4; ++mb) acc[mb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                    for (int q = 0; q < NBG; ++q)
+#pragma unroll
+                        for (int mb = 0; mb < 4; ++mb) acc[q][mb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
                     // K in two halves: the weight columns of lanes 0-31, then 32-63
                     for (int h = 0; h < 2; ++h) {
                         const bool mine = in_r && (lane >> 5) == h;
@@ -962,15 +1105,18 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                             for (int k = 0; k < 8; ++k) wt[wofs[k]] = w[k];
                         }
                         wave_lds_order();
-                        const float4 *sb = reinterpret_cast<const float4 *>(
-                            gt + (16 * nb + m) * kGradStride + kHalf * h + 8 * g);
-                        const float4 b0 = sb[0], b1 = sb[1];
 #pragma unroll
-                        for (int mb = 0; mb < 4; ++mb) {
-                            if (mb < nmb) {
-                                const float4 *sa =
-                                    reinterpret_cast<const float4 *>(wt + (16 * mb + m) * kHalf);
-                                acc[mb] = mfma4(sa[k1], b1, mfma4(sa[k0], b0, acc[mb]));
+                        for (int q = 0; q < NBG; ++q) {
+                            const float4 *sb = reinterpret_cast<const float4 *>(
+                                gt + (16 * (nb0 + q) + m) * kGradStride + kHalf * h + 8 * g);
+                            const float4 b0 = sb[0], b1 = sb[1];
+#pragma unroll
+                            for (int mb = 0; mb < 4; ++mb) {
+                                if (mb < nmb) {
+                                    const float4 *sa =
+                                        reinterpret_cast<const float4 *>(wt + (16 * mb + m) * kHalf);
+                                    acc[q][mb] = mfma4(sa[k1], b1, mfma4(sa[k0], b0, acc[q][mb]));
+                                }
                             }
                         }
                         wave_lds_order();
@@ -979,7 +1125,6 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                             for (int k = 0; k < 8; ++k) wt[wofs[k]] = 0.0f;
                         }
                     }
-                    const int ch = 16 * nb + m;
 #pragma unroll
                     for (int mb = 0; mb < 4; ++mb) {
                         if (mb < nmb) {
@@ -987,9 +1132,13 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                             for (int i = 0; i < 4; ++i) {
                                 const int v = 16 * mb + 4 * g + i;
                                 const int vo = __shfl(vofs, v, kWave);
-                                const float d = acc[mb][i];
-                                if (v < nv && ch < C && d != 0.0f)
-                                    atomic_addf(gdata + (vbase + vo) * C + ch, d);
+#pragma unroll
+                                for (int q = 0; q < NBG; ++q) {
+                                    const int ch = 16 * (nb0 + q) + m;
+                                    const float d = acc[q][mb][i];
+                                    if (v < nv && ch < C && d != 0.0f)
+                                        atomic_addf(gdata + (vbase + vo) * C + ch, d);
+                                }
                             }
                         }
                     }
@@ -1007,6 +1156,11 @@ template <int NC>
 constexpr size_t tile_lds_bytes() {
     return (size_t)(tile_channels<NC>() * kGradStride + kCapV * kHalf + 7 * kQBoxCells) *
            sizeof(float);
+}
+
+template <int NC>
+constexpr size_t fwd_lds_bytes() {
+    return 3 * NC + 5 == 32 ? (size_t)kWavesPerBlock * kBoxV * (3 * NC + 5) * sizeof(float) : 0;
 }
 
 static int64_t fused_waves(uint32_t W, uint32_t H, int n_cams, const int32_t *ray_pixel,
@@ -1146,7 +1300,7 @@ static int fused_forward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t 
     int64_t waves = fused_waves(width, height, n_cams, ray_pixel, n_rays);
     unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
 #define QSDVR_FWD(NC, BINS)                                                                  \
-    fused_fwd_kernel<NC, BINS><<<blocks, kFusedBlock, 0, s>>>(                               \
+    fused_fwd_kernel<NC, BINS><<<blocks, kFusedBlock, fwd_lds_bytes<NC>(), s>>>(             \
         cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid, data_grid, \
         reso, interval, logistic_coef, color, depth, rs, tlist, K, bins)
     if (channels == 32) {
