@@ -79,6 +79,14 @@ __device__ __forceinline__ void store3(float *p, int64_t i, f3 v) {
     q[2] = v.z;
 }
 
+// Hardware reciprocal, reciprocal square root and square root (v_rcp_f32,
+// v_rsq_f32, v_sqrt_f32: about 1 ulp).  The library is built with IEEE
+// division and square root for the bit-exact ray generation; the per-sample
+// shading and compositing math, held to relative tolerances, uses these.
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Hardware float atomic add (global_atomic_add_f32, no return value).
 __device__ __forceinline__ void atomic_addf(float *p, float v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

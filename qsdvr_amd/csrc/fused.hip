@@ -162,6 +162,42 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+// DPP lane moves within rows of 16 lanes (row_shr:D = 0x110 + D, row_shl:1
+// = 0x101, quad_perm, mirrors); a lane whose source lies outside its row
+// reads 0
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+
+// Wave-wide min / max returned in a scalar register: DPP steps within each
+// row of 16 lanes (quad_perm xor 1 and xor 2, half-row mirror, row mirror)
+// leave every lane with its row's extreme; the four rows are read with
+// readlane.  Every lane of the wave must be active.
+__device__ __forceinline__ int wave_min_s(int v) {
+    v = min(v, dpp_i<0xB1>(v));
+    v = min(v, dpp_i<0x4E>(v));
+    v = min(v, dpp_i<0x141>(v));
+    v = min(v, dpp_i<0x140>(v));
+    const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+    const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return min(min(a, b), min(c, d));
+}
+
+__device__ __forceinline__ int wave_max_s(int v) {
+    v = max(v, dpp_i<0xB1>(v));
+    v = max(v, dpp_i<0x4E>(v));
+    v = max(v, dpp_i<0x141>(v));
+    v = max(v, dpp_i<0x140>(v));
+    const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+    const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return max(max(a, b), max(c, d));
+}
+
 // Orders this wave's LDS accesses for the compiler only.  A wave's LDS
 // instructions execute in issue order, so LDS data written by one lane is
 // visible to a later read by another lane of the same wave without
@@ -172,33 +208,30 @@ __device__ __forceinline__ void wave_lds_order() { __asm__ volatile("" ::: "memo
 // samples fall in at one step, copied to LDS once by the whole wave (64
 // lanes x 16 B per load) and read there by each lane, in place of 64
 // per-lane 16-B gathers per sample.  kBoxV vertices of C = 32 channels per
-// wave (8 KB); the quads of vertex v are XOR-swizzled by v's low bits, so
-// lanes reading different vertices at one quad index hit different banks.
+// wave, each row padded by one quad (9 KB): a lane's 8 reads of a row share
+// one address register (the quad is the instruction's offset), and lanes
+// reading different vertices at one quad fall in different banks.
 constexpr int kBoxV = 64;
 
 struct BoxView {
     const float4 *lds;  // nullptr: gather from global memory
-    int vb, ex, sxy;    // the lane's cell-origin vertex in the box, box strides
+    int vb;             // the lane's cell-origin vertex in the box
+    int ex, sxy;        // box strides in vertices
 };
-
-template <int Q>
-__device__ __forceinline__ int box_slot(int v, int q) {
-    static_assert((Q & (Q - 1)) == 0, "quads per row must be a power of two");
-    return v * Q + (q ^ (v & (Q - 1)));
-}
 
 // gather_interp from the staged box (same operation order).
 template <int C>
 __device__ __forceinline__ void box_interp(const BoxView &bx, const float w[8], float d[C]) {
-    constexpr int Q = C / 4;
+    constexpr int Q = C / 4, QP = Q + 1;
 #pragma unroll
     for (int c = 0; c < C; ++c) d[c] = 0.0f;
+    const float4 *p0 = bx.lds + bx.vb * QP;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const int v = bx.vb + (k & 1) + ((k >> 1) & 1) * bx.ex + (k >> 2) * bx.sxy;
+        const float4 *p = p0 + ((k & 1) + ((k >> 1) & 1) * bx.ex + (k >> 2) * bx.sxy) * QP;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const float4 val = bx.lds[box_slot<Q>(v, q)];
+            const float4 val = p[q];
             d[4 * q + 0] += w[k] * val.x;
             d[4 * q + 1] += w[k] * val.y;
             d[4 * q + 2] += w[k] * val.z;
@@ -210,26 +243,30 @@ __device__ __forceinline__ void box_interp(const BoxView &bx, const float w[8], 
 // Stage the box of the cells of the lanes with ``need`` set (their cell
 // ``rp``): returns the lane's view of it, or a null view (gather from global
 // memory) when the box has more than kBoxV vertices.  Called by the whole
-// wave (wave-uniform control flow).
+// wave with every lane active.
 template <int C>
 __device__ __forceinline__ BoxView stage_box(float4 *box, const float *__restrict__ data, int R,
                                              bool need, const CellPoint &rp, int lane) {
-    constexpr int Q = C / 4;
-    int lx = need ? rp.ix : INT_MAX, ly = need ? rp.iy : INT_MAX, lz = need ? rp.iz : INT_MAX;
-    int hx = need ? rp.ix : INT_MIN, hy = need ? rp.iy : INT_MIN, hz = need ? rp.iz : INT_MIN;
-    lx = wave_min(lx); ly = wave_min(ly); lz = wave_min(lz);
-    hx = wave_max(hx); hy = wave_max(hy); hz = wave_max(hz);
+    constexpr int Q = C / 4, QP = Q + 1;
+    const int lx = wave_min_s(need ? rp.ix : INT_MAX), hx = wave_max_s(need ? rp.ix : INT_MIN);
+    const int ly = wave_min_s(need ? rp.iy : INT_MAX), hy = wave_max_s(need ? rp.iy : INT_MIN);
+    const int lz = wave_min_s(need ? rp.iz : INT_MAX), hz = wave_max_s(need ? rp.iz : INT_MIN);
     const int ex = hx - lx + 2, ey = hy - ly + 2, ez = hz - lz + 2;
     const int sxy = ex * ey, nv = sxy * ez;
     if (nv > kBoxV) return BoxView{nullptr, 0, 0, 0};
+    // box vertex v = vx + vy ex + vz sxy (v < 64, sxy >= 4, ex >= 2): the
+    // quotients from float reciprocals are exact, since (v + 0.5) / sxy lies
+    // at least 0.5 / sxy from an integer
+    const float rsxy = __builtin_amdgcn_rcpf((float)sxy), rex = __builtin_amdgcn_rcpf((float)ex);
     const int64_t RR = (int64_t)R * R;
-    const float4 *__restrict__ src = reinterpret_cast<const float4 *>(data);
+    const float4 *__restrict__ src =
+        reinterpret_cast<const float4 *>(data) + ((int64_t)lx + (int64_t)ly * R + lz * RR) * Q;
     wave_lds_order();  // the previous step's box reads are issued before these writes
     for (int i = lane; i < nv * Q; i += kWave) {
         const int v = i / Q, q = i % Q;
-        const int vz = v / sxy, r2 = v - vz * sxy, vy = r2 / ex, vx = r2 - vy * ex;
-        const int64_t row = (int64_t)(lx + vx) + (int64_t)(ly + vy) * R + (int64_t)(lz + vz) * RR;
-        box[box_slot<Q>(v, q)] = src[row * Q + q];
+        const int vz = (int)(((float)v + 0.5f) * rsxy), r2 = v - vz * sxy;
+        const int vy = (int)(((float)r2 + 0.5f) * rex), vx = r2 - vy * ex;
+        box[v * QP + q] = src[(vx + vy * R + vz * (int)RR) * Q + q];
     }
     wave_lds_order();
     return BoxView{box, (rp.ix - lx) + (rp.iy - ly) * ex + (rp.iz - lz) * sxy, ex, sxy};
@@ -290,7 +327,7 @@ __device__ __forceinline__ void eval_render_at(const RaySetup &ray, uint32_t j, 
     const float px = ray.ori.x + ray.dir.x * sj - ray.center.x;
     const float py = ray.ori.y + ray.dir.y * sj - ray.center.y;
     const float pz = ray.ori.z + ray.dir.z * sj - ray.center.z;
-    e.t = sqrtf(px * px + py * py + pz * pz);
+    e.t = fast_sqrt(px * px + py * py + pz * pz);
 }
 
 // Render sample j with per-lane gathers from global memory.
@@ -315,7 +352,7 @@ __device__ __forceinline__ void eval_render_sample(const RaySetup &ray, uint32_t
     const float px = ray.ori.x + ray.dir.x * sj - ray.center.x;
     const float py = ray.ori.y + ray.dir.y * sj - ray.center.y;
     const float pz = ray.ori.z + ray.dir.z * sj - ray.center.z;
-    e.t = sqrtf(px * px + py * py + pz * pz);
+    e.t = fast_sqrt(px * px + py * py + pz * pz);
 }
 
 __device__ __forceinline__ float sdf_at(const RaySetup &ray, uint32_t j, float interval,
@@ -344,7 +381,7 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
     // wave-uniformly (lanes past their last sample idle) for the staging.
     const bool stage = C == 32 && ray_pixel == nullptr;
     const int lane = threadIdx.x % kWave;
-    float4 *box = fwd_lds + (threadIdx.x / kWave) * (kBoxV * C / 4);
+    float4 *box = fwd_lds + (threadIdx.x / kWave) * (kBoxV * (C / 4 + 1));
     const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
     f3 col{0.0f, 0.0f, 0.0f};
     float dep = 0.0f;
@@ -362,7 +399,7 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
         float a = 0.0f, wt = 0.0f, e1 = 0.0f;
         if (act) {
             e1 = expf(-s * sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp));
-            a = fmaxf((e1 - e0) / (1.0f + e1), 0.0f);
+            a = fmaxf((e1 - e0) * fast_rcp(1.0f + e1), 0.0f);
             wt = a * T;
         }
         // A sample of weight exactly zero (alpha clamped to 0 where the SDF
@@ -445,7 +482,7 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_kernel(
     for (uint32_t j = 0; j < ray.renders; ++j) {
         float sdf1 = sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp1);
         float e1 = expf(-s * sdf1);
-        float inv1 = 1.0f / (1.0f + e1);
+        float inv1 = fast_rcp(1.0f + e1);
         float praw = (e1 - e0) * inv1;
         float a = fmaxf(praw, 0.0f);
         const float wt = a * T;
@@ -467,7 +504,7 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_kernel(
         // dL/dalpha_j = T_j (gC.rgb_j + gD t_j) - (gC.S_j + gD SD_j) / (1 - alpha_j)
         float g_sdf0 = carry, g_sdf1 = 0.0f;
         if (praw >= 0.0f) {
-            float one_over = (1.0f + e1) / (1.0f + e0);
+            float one_over = (1.0f + e1) * fast_rcp(1.0f + e0);
             float suffix = gC.x * (Cs.x - pre.x) + gC.y * (Cs.y - pre.y) + gC.z * (Cs.z - pre.z) +
                            gD * (Ds - pre_d);
             float da = T * here - suffix * one_over;
@@ -568,7 +605,7 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
         const uint32_t q = base + j;
         float sdf1 = sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp1);
         float e1 = expf(-s * sdf1);
-        float inv1 = 1.0f / (1.0f + e1);
+        float inv1 = fast_rcp(1.0f + e1);
         float praw = (e1 - e0) * inv1;
         float a = fmaxf(praw, 0.0f);
         const float wt = a * T;
@@ -586,7 +623,7 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
         }
         float g_sdf0 = carry, g_sdf1 = 0.0f;
         if (praw >= 0.0f) {
-            float one_over = (1.0f + e1) / (1.0f + e0);
+            float one_over = (1.0f + e1) * fast_rcp(1.0f + e0);
             float suffix = gC.x * (Cs.x - pre.x) + gC.y * (Cs.y - pre.y) + gC.z * (Cs.z - pre.z) +
                            gD * (Ds - pre_d);
             float da = T * here - suffix * one_over;
@@ -673,17 +710,6 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
 // along its longest axis, or 3x3x3-cell boxes (4x4x4 = 64 vertices).
 constexpr int kCapV = 64;
 static_assert(kCapV >= 64, "a fallback round's 3x3x3-cell box has 64 vertices");
-
-// DPP lane shifts within rows of 16 lanes (row_shr:D = 0x110 + D, row_shl:1
-// = 0x101); a lane whose source lies outside its row reads 0
-template <int CTRL>
-__device__ __forceinline__ int dpp_i(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
-}
-template <int CTRL>
-__device__ __forceinline__ float dpp_f(float v) {
-    return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
-}
 
 // One step of a segmented inclusive scan over the 8-lane row: lane x adds
 // the partial sum of lane x - D unless a run starts in (x - D, x].
@@ -954,7 +980,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
         if (act_r) {
             float sdf1 = sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp1);
             float e1 = expf(-s * sdf1);
-            float inv1 = 1.0f / (1.0f + e1);
+            float inv1 = fast_rcp(1.0f + e1);
             float praw = (e1 - e0) * inv1;
             float a = fmaxf(praw, 0.0f);
             const float wt = a * T;
@@ -970,7 +996,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                 pre.y += ev.rgb.y * wt;
                 pre.z += ev.rgb.z * wt;
                 pre_d += ev.t * wt;
-                float one_over = (1.0f + e1) / (1.0f + e0);
+                float one_over = (1.0f + e1) * fast_rcp(1.0f + e0);
                 float suffix = gC.x * (Cs.x - pre.x) + gC.y * (Cs.y - pre.y) +
                                gC.z * (Cs.z - pre.z) + gD * (Ds - pre_d);
                 float here = gC.x * ev.rgb.x + gC.y * ev.rgb.y + gC.z * ev.rgb.z + gD * ev.t;
@@ -989,7 +1015,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                 }
             } else if (praw >= 0.0f) {
                 // T == 0: dL/dalpha = -suffix / (1 - alpha)
-                float one_over = (1.0f + e1) / (1.0f + e0);
+                float one_over = (1.0f + e1) * fast_rcp(1.0f + e0);
                 float suffix = gC.x * (Cs.x - pre.x) + gC.y * (Cs.y - pre.y) +
                                gC.z * (Cs.z - pre.z) + gD * (Ds - pre_d);
                 float da = -suffix * one_over;
@@ -1160,7 +1186,7 @@ constexpr size_t tile_lds_bytes() {
 
 template <int NC>
 constexpr size_t fwd_lds_bytes() {
-    return 3 * NC + 5 == 32 ? (size_t)kWavesPerBlock * kBoxV * (3 * NC + 5) * sizeof(float) : 0;
+    return 3 * NC + 5 == 32 ? (size_t)kWavesPerBlock * kBoxV * ((3 * NC + 5) / 4 + 1) * 16 : 0;
 }
 
 static int64_t fused_waves(uint32_t W, uint32_t H, int n_cams, const int32_t *ray_pixel,
@@ -1258,7 +1284,7 @@ __global__ void __launch_bounds__(256) plain_bins_kernel(const float4 *__restric
                     clamp_cell(cp, R);
                     e_next = expf(-s * quadric_sdf(load_quadric(qgrid, cp.cell), cp.local, inv_reso));
                 }
-                const float a = valid ? fmaxf((e_next - e) / (1.0f + e_next), 0.0f) : 0.0f;
+                const float a = valid ? fmaxf((e_next - e) * fast_rcp(1.0f + e_next), 0.0f) : 0.0f;
                 // inclusive product scan of (1 - a) over the lanes
                 float p = 1.0f - a;
 #pragma unroll

@@ -23,8 +23,7 @@ __device__ __forceinline__ float quadric_sdf(const Quadric &q, f3 p, float inv_r
     float a0 = q.a * p.x, a1 = q.b * p.y, a2 = q.c * p.z;
     float f = (a0 + q.d) * p.x + (a1 + q.e) * p.y + (a2 + q.f) * p.z + q.g;
     float gx = 2.0f * a0 + q.d, gy = 2.0f * a1 + q.e, gz = 2.0f * a2 + q.f;
-    float n = sqrtf(gx * gx + gy * gy + gz * gz);
-    return f / n * inv_reso;
+    return f * fast_rsqrt(gx * gx + gy * gy + gz * gz) * inv_reso;
 }
 
 // Adds d sdf / d(A..G) * upstream to g[7] (closed form of the quotient rule):
@@ -35,7 +34,7 @@ __device__ __forceinline__ void quadric_sdf_grad(const Quadric &q, f3 p, float u
     float a0 = q.a * p.x, a1 = q.b * p.y, a2 = q.c * p.z;
     float f = (a0 + q.d) * p.x + (a1 + q.e) * p.y + (a2 + q.f) * p.z + q.g;
     float gx = 2.0f * a0 + q.d, gy = 2.0f * a1 + q.e, gz = 2.0f * a2 + q.f;
-    float k = 1.0f / sqrtf(gx * gx + gy * gy + gz * gz);
+    float k = fast_rsqrt(gx * gx + gy * gy + gz * gz);
     float qq = f * k * k * k;
     float s = upstream * inv_reso;
     float tx = p.x * k - gx * qq, ty = p.y * k - gy * qq, tz = p.z * k - gz * qq;
@@ -51,8 +50,8 @@ __device__ __forceinline__ void quadric_sdf_grad(const Quadric &q, f3 p, float u
 // n = grad f / max(||grad f||, 1e-12)   (F.normalize, render_torch.py:100)
 __device__ __forceinline__ f3 quadric_normal(const Quadric &q, f3 p, float *inv_len = nullptr) {
     float gx = 2.0f * q.a * p.x + q.d, gy = 2.0f * q.b * p.y + q.e, gz = 2.0f * q.c * p.z + q.f;
-    float n = fmaxf(sqrtf(gx * gx + gy * gy + gz * gz), 1e-12f);
-    float k = 1.0f / n;
+    // 1 / max(||g||, 1e-12): rsq(0) = inf clamps to 1e12
+    float k = fminf(fast_rsqrt(gx * gx + gy * gy + gz * gz), 1e12f);
     if (inv_len) *inv_len = k;
     return f3{gx * k, gy * k, gz * k};
 }
@@ -61,10 +60,10 @@ __device__ __forceinline__ f3 quadric_normal(const Quadric &q, f3 p, float *inv_
 // h = (nbar - n (n . nbar)) / ||grad f||; dA = 2x h_x, dD = h_x, ...
 __device__ __forceinline__ void quadric_normal_grad(const Quadric &q, f3 p, f3 nbar, float g[7]) {
     float gx = 2.0f * q.a * p.x + q.d, gy = 2.0f * q.b * p.y + q.e, gz = 2.0f * q.c * p.z + q.f;
-    float len = sqrtf(gx * gx + gy * gy + gz * gz);
+    float len2 = gx * gx + gy * gy + gz * gz;
     float hx, hy, hz;
-    if (len > 1e-12f) {
-        float k = 1.0f / len;
+    if (len2 > 1e-24f) {  // ||grad f|| > 1e-12
+        float k = fast_rsqrt(len2);
         float nx = gx * k, ny = gy * k, nz = gz * k;
         float dn = nx * nbar.x + ny * nbar.y + nz * nbar.z;
         hx = (nbar.x - nx * dn) * k;
