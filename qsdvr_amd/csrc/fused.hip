@@ -1050,8 +1050,8 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                 lx = rp.ix; ly = rp.iy; lz = rp.iz;
                 hx = rp.ix; hy = rp.iy; hz = rp.iz;
             }
-            lx = wave_min(lx); ly = wave_min(ly); lz = wave_min(lz);
-            hx = wave_max(hx); hy = wave_max(hy); hz = wave_max(hz);
+            lx = wave_min_s(lx); ly = wave_min_s(ly); lz = wave_min_s(lz);
+            hx = wave_max_s(hx); hy = wave_max_s(hy); hz = wave_max_s(hz);
             if (hx < lx) break;  // nothing pending (wave-uniform)
             const int cx = hx - lx + 1, cy = hy - ly + 1, cz = hz - lz + 1;  // cells
             if ((cx + 1) * (cy + 1) * (cz + 1) > kCapV) {
