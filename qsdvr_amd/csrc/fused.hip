@@ -699,8 +699,8 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
 // global atomic per sample and channel was ~40x slower still.)  The 13
 // quadric gradients per sample (7 at the sdf sample's cell, 6 at the render
 // sample's) are carried per lane while its ray stays in a cell, summed over
-// equal cells across the wave (DPP runs along each 8-lane pixel row, then a
-// butterfly over rows) and added with LDS atomics into a 4x4x4-cell window
+// runs of equal cells along each 8-lane pixel row (DPP scans) and added
+// with LDS atomics into a 4x4x4-cell window
 // that goes to the grid with global atomics only when the rays leave it:
 // per-lane LDS atomics without the merge serialised on the same words, and
 // per-step global atomics held up the next step's gathers.  Steps whose
@@ -726,11 +726,12 @@ __device__ __forceinline__ void run_scan_step(float v[7], bool &f, int x) {
     }
 }
 
-// Sums the first N of v[] over the lanes with equal `key` (key < 0: no
-// value): first over runs of equal keys along each 8-lane pixel row, then
-// over such runs in different rows.  Returns the key whose sum this lane
-// now holds in v[] (one lane per distinct key), or -1.  Must be called by
-// all 64 lanes.
+// Sums the first N of v[] over runs of equal `key` (key < 0: no value)
+// along each 8-lane pixel row.  Returns the key whose sum this lane now
+// holds in v[] (the last lane of each run), or -1.  Equal keys in different
+// rows stay separate sums (at most one per row): the window adds them with
+// LDS atomics, which measured faster than merging them across rows with a
+// shuffle butterfly.  Must be called by all 64 lanes.
 template <int N>
 __device__ __forceinline__ int row_run_merge(float v[7], int key, int lane) {
     const int x = lane & 7;
@@ -741,26 +742,7 @@ __device__ __forceinline__ int row_run_merge(float v[7], int key, int lane) {
     run_scan_step<N, 1>(v, f, x);
     run_scan_step<N, 2>(v, f, x);
     run_scan_step<N, 4>(v, f, x);
-    // the last lane of each run holds its sum; merge the runs of equal keys
-    // in different rows (butterfly over the row bits: of two partners with
-    // the same key the lower lane takes the sum, the upper drops out)
-    int tail = key >= 0 && (x == 7 || next_head) ? key : -1;
-#pragma unroll
-    for (int m = 8; m < kWave; m <<= 1) {
-        const int pk = __shfl_xor(tail, m, kWave);
-        float p[N];
-#pragma unroll
-        for (int k = 0; k < N; ++k) p[k] = __shfl_xor(v[k], m, kWave);
-        if (tail >= 0 && pk == tail) {
-            if (lane & m) {
-                tail = -1;
-            } else {
-#pragma unroll
-                for (int k = 0; k < N; ++k) v[k] += p[k];
-            }
-        }
-    }
-    return tail;
+    return key >= 0 && (x == 7 || next_head) ? key : -1;
 }
 
 // Quadric gradient box: a 4x4x4-cell window of the quadric gradient grid
@@ -818,8 +800,8 @@ __device__ __forceinline__ void qbox_add(float *__restrict__ qb, QuadricBox &b, 
              (unsigned)(ty - b.by) < (unsigned)kQBox && (unsigned)(tz - b.bz) < (unsigned)kQBox;
     }
     if (in) {
-        // LDS float atomics: the merge leaves at most a few lanes per cell
-        // (equal cells whose row runs end at different row positions)
+        // LDS float atomics: the row merge leaves at most one lane per cell
+        // and pixel row
         const int slot = (tx - b.bx) + kQBox * (ty - b.by) + kQBox * kQBox * (tz - b.bz);
 #pragma unroll
         for (int k = 0; k < 7; ++k)
