@@ -1074,7 +1074,10 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
             // box-vertex offsets in the global grids, vertex v held by lane v
             int vofs = 0;
             if (lane < nv) {
-                const int vz = lane / sxy, r2 = lane - vz * sxy, vy = r2 / ex, vx = r2 - vy * ex;
+                // exact quotients (lane < 64, ex >= 2, sxy >= 4), as in stage_box
+                const int vz = (int)(((float)lane + 0.5f) * fast_rcp((float)sxy));
+                const int r2 = lane - vz * sxy;
+                const int vy = (int)(((float)r2 + 0.5f) * fast_rcp((float)ex)), vx = r2 - vy * ex;
                 vofs = vx + vy * R + vz * (int)RR;
             }
             const int64_t vbase = (int64_t)lx + (int64_t)ly * R + (int64_t)lz * RR;
