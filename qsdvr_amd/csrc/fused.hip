@@ -130,8 +130,19 @@ __device__ __forceinline__ void gather_interp(const float *__restrict__ data, in
                 d[4 * q + 3] += w[k] * v.w;
             }
         } else {
+            // rows of C floats are only 4-B aligned: 16-B loads at 4-B alignment
+            // (global_load_dwordx4 needs dword alignment), then the tail
+            typedef float f4a __attribute__((ext_vector_type(4), aligned(4)));
 #pragma unroll
-            for (int q = 0; q < C; ++q) d[q] += w[k] * src[q];
+            for (int q = 0; q < C / 4; ++q) {
+                const f4a v = *reinterpret_cast<const f4a *>(src + 4 * q);
+                d[4 * q + 0] += w[k] * v.x;
+                d[4 * q + 1] += w[k] * v.y;
+                d[4 * q + 2] += w[k] * v.z;
+                d[4 * q + 3] += w[k] * v.w;
+            }
+#pragma unroll
+            for (int q = C / 4 * 4; q < C; ++q) d[q] += w[k] * src[q];
         }
     }
 }
