@@ -22,16 +22,18 @@ int bits_above(uint64_t n) {
     return b;
 }
 
+// Temporary-storage size queries.  A failed query leaves 0; the checked
+// call that later uses the storage then reports the error.
 size_t scan_temp_bytes(int64_t n) {
     size_t t = 0;
-    rocprim::exclusive_scan(nullptr, t, (const uint32_t *)nullptr, (uint32_t *)nullptr, 0u,
+    (void)rocprim::exclusive_scan(nullptr, t, (const uint32_t *)nullptr, (uint32_t *)nullptr, 0u,
                             (size_t)n, rocprim::plus<uint32_t>(), (hipStream_t)0);
     return t;
 }
 
 size_t sort_temp_bytes(int64_t n, int end_bit) {
     size_t t = 0;
-    rocprim::radix_sort_pairs(nullptr, t, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+    (void)rocprim::radix_sort_pairs(nullptr, t, (const uint32_t *)nullptr, (uint32_t *)nullptr,
                               (const uint32_t *)nullptr, (uint32_t *)nullptr, (size_t)n, 0u,
                               (unsigned)end_bit, (hipStream_t)0);
     return t;
