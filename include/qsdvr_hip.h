@@ -221,12 +221,14 @@ int qsdvr_ray_aggregate_backward(const float *grad_out, const float *rgb,
  * arguments and the same result (added into the caller-zeroed grid) as
  * qsdvr_grid_interpolation_backward (kernel/render.cu:325-331),
  * qsdvr_sample_sdf_backward (kernel/surface.cu:352-359) and
- * qsdvr_sample_normal_backward (kernel/surface.cu:369-375).  The sample
- * indices are radix-sorted by cell and every cell's samples are summed in
- * sample order before one read-modify-write of the cell's rows (the
- * appearance in 8 cell-parity passes), instead of one float atomic per
- * sample and coefficient: bitwise reproducible.  Workspace:
- * qsdvr_sorted_scatter_workspace_size(n, cells) bytes with cells = R^3
+ * qsdvr_sample_normal_backward (kernel/surface.cu:369-375).  The samples
+ * are radix-sorted by cell (SampleSDF sorts its point and upstream gradient
+ * with the keys, the others sample indices) and every cell's samples are
+ * summed in a fixed order before one read-modify-write of the cell's rows
+ * (the appearance in 8 cell-parity passes), instead of one float atomic
+ * per sample and coefficient: bitwise reproducible.  Workspace:
+ * qsdvr_sorted_scatter_workspace_size(n, cells) bytes (about 41 B per
+ * sample plus the sort's temporary storage) with cells = R^3
  * (cell_count for the quadric ops, (vertex_reso - 1)^3 for the
  * interpolation).  GridInterpolation: channels 32 or 53 and an even cell
  * resolution (with the stride-R vertex rows of the reference, cells of one

@@ -45,6 +45,7 @@ struct DetBuffers {
     float *aval = nullptr;        // [cap][det_app_stride(C)]
     uint32_t *bounds = nullptr;   // [9]: class boundaries in the sorted appearance keys
     uint32_t *total = nullptr;    // [1] host-visible copy target is the caller's
+    float *qhead = nullptr, *qtail = nullptr;  // [waves of 2 cap][7] quadric segment sums
     void *temp = nullptr;         // rocPRIM temporary storage
     size_t temp_bytes = 0;
 };

@@ -15,6 +15,9 @@ namespace qsdvr {
 
 namespace {
 
+// waves of 64 records in the quadric segmented sum (head / tail sums)
+int64_t seg_waves(int64_t n) { return (n + kWave - 1) / kWave; }
+
 // smallest b with 2^b > n
 int bits_above(uint64_t n) {
     int b = 1;
@@ -65,6 +68,8 @@ size_t det_workspace_bytes(int64_t n_rays, int64_t cap, int C, int R) {
     carve<float>(c, cap * det_app_stride(C));
     carve<uint32_t>(c, 16);
     carve<uint32_t>(c, 1);
+    carve<float>(c, 7 * seg_waves(2 * cap));
+    carve<float>(c, 7 * seg_waves(2 * cap));
     carve<char>(c, temp_bytes_for(n_rays, cap, R));
     return c.used + 256;
 }
@@ -97,6 +102,8 @@ int det_carve(void *ws, size_t bytes, int64_t n_rays, int64_t cap, int C, int R,
     b.aval = carve<float>(c, cap * det_app_stride(C));
     b.bounds = carve<uint32_t>(c, 16);
     b.total = carve<uint32_t>(c, 1);
+    b.qhead = carve<float>(c, 7 * seg_waves(2 * cap));
+    b.qtail = carve<float>(c, 7 * seg_waves(2 * cap));
     b.temp_bytes = temp_bytes_for(n_rays, cap, R);
     b.temp = carve<char>(c, b.temp_bytes);
     if (ws == nullptr || b.temp == nullptr) {
@@ -156,14 +163,17 @@ __device__ __forceinline__ int64_t run_end(const uint32_t *__restrict__ key, int
     }
 }
 
-// Where a quadric run's record r gets its 7 coefficient gradients from:
-// the adjoint's precomputed records, or (op surface) the closed forms of the
+// Where the record at sorted position `pos` of a quadric run gets its 7
+// coefficient gradients from: the adjoint's precomputed records (through
+// the sorted record indices), or (op surface) the closed forms of the
 // SampleSDF / SampleNormal adjoints at the sample's point and upstream
-// gradient, with the cell's quadric.
+// gradient, sorted along with the keys, with the cell's quadric.
 struct QuadricRecords {
     const float4 *qval;
-    __device__ __forceinline__ void add(uint32_t r, uint32_t, float acc[7]) const {
-        const float4 a = qval[2 * (int64_t)r], c = qval[2 * (int64_t)r + 1];
+    const uint32_t *idx;
+    __device__ __forceinline__ void add(int64_t pos, uint32_t, float acc[7]) const {
+        const int64_t r = idx[pos];
+        const float4 a = qval[2 * r], c = qval[2 * r + 1];
         acc[0] += a.x;
         acc[1] += a.y;
         acc[2] += a.z;
@@ -174,65 +184,128 @@ struct QuadricRecords {
     }
 };
 
+// SampleSDF record: local point and upstream gradient
 struct SdfSamples {
-    const float *grid, *pts, *up;
+    const float4 *pay;
+    const float *grid;
     float inv_reso;
-    __device__ __forceinline__ void add(uint32_t r, uint32_t cell, float acc[7]) const {
-        quadric_sdf_grad(load_quadric(grid, cell), load3(pts, r), up[r], inv_reso, acc);
+    __device__ __forceinline__ void add(int64_t pos, uint32_t cell, float acc[7]) const {
+        const float4 p = pay[pos];
+        quadric_sdf_grad(load_quadric(grid, cell), f3{p.x, p.y, p.z}, p.w, inv_reso, acc);
     }
 };
 
+// SampleNormal record: gathered through the sorted sample index (sorting
+// the 24-B point and upstream gradient with the keys measured slower than
+// gathering them)
 struct NormalSamples {
+    const uint32_t *idx;
     const float *grid, *pts, *up;
-    __device__ __forceinline__ void add(uint32_t r, uint32_t cell, float acc[7]) const {
+    __device__ __forceinline__ void add(int64_t pos, uint32_t cell, float acc[7]) const {
+        const int64_t r = idx[pos];
         quadric_normal_grad(load_quadric(grid, cell), load3(pts, r), load3(up, r), acc);
     }
 };
 
-// Quadric runs.  One thread per sorted position; the thread at the head of
-// a run of at most 64 records sums it alone, in order, and adds the sum to
-// its cell.  Longer runs (a cell crossed by many rays: full frames, and the
-// op surface's per-pose sample lists) are summed by the whole wave, lanes
-// striding over the records and a fixed butterfly combining them.
+// Quadric runs: a deterministic segmented sum in two passes.  Pass 1: wave
+// w takes sorted positions [64 w, 64 w + 64), one record per lane, and a
+// segmented inclusive scan over equal keys (Hillis-Steele, a fixed tree
+// order) leaves each segment's sum in its last lane.  A run that lies
+// inside the wave is added to its cell there; the part of a run that enters
+// from the previous wave is stored as the wave's head sum, the part of a
+// run that starts in the wave and leaves it as its tail sum.  Pass 2: the
+// wave where such a run starts adds its tail sum and the head sums of the
+// waves the run covers, in order, to the cell.  The same input therefore
+// always gives the same bits.
 template <class Src>
-__global__ void __launch_bounds__(256) det_quadric_reduce_kernel(
-    const uint32_t *__restrict__ key, const uint32_t *__restrict__ idx, Src src, int64_t n,
-    uint32_t invalid, float *__restrict__ gq) {
+__global__ void __launch_bounds__(256) quadric_seg_kernel(const uint32_t *__restrict__ key,
+                                                          Src src, int64_t n, uint32_t invalid,
+                                                          float *__restrict__ gq,
+                                                          float *__restrict__ head,
+                                                          float *__restrict__ tail) {
     const int lane = threadIdx.x % kWave;
-    const int64_t p0 = ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave;
-    if (p0 >= n) return;
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int64_t p0 = w * kWave;
+    if (p0 >= n) return;  // wave-uniform
     const int64_t p = p0 + lane;
     const uint32_t k = p < n ? key[p] : invalid;
-    const bool head = p < n && k != invalid && (p == 0 || key[p - 1] != k);
-    const bool long_run = head && p + kWave < n && key[p + kWave] == k;
-    if (head && !long_run) {
-        float acc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        for (int64_t i = p; i < n && key[i] == k; ++i) src.add(idx[i], k, acc);
-        float *dst = gq + 7 * (int64_t)k;
+    float v[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (k != invalid) src.add(p, k, v);
+    const uint32_t kp = __shfl_up(k, 1, kWave), kn = __shfl_down(k, 1, kWave);
+    const bool start = lane == 0 || kp != k;
+    const bool last = lane == kWave - 1 || kn != k;
+    const unsigned long long starts = __ballot(start);
+    bool f = start;
 #pragma unroll
-        for (int j = 0; j < 7; ++j) dst[j] += acc[j];
-    }
-    unsigned long long longs = __ballot(long_run);
-    while (longs != 0) {
-        const int h = __ffsll((long long)longs) - 1;
-        longs &= longs - 1;
-        const uint32_t hk = __shfl(k, h, kWave);
-        const int64_t start = p0 + h;
-        const int64_t end = run_end(key, start + kWave, n, hk);
-        float acc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        for (int64_t i = start + lane; i < end; i += kWave) src.add(idx[i], hk, acc);
+    for (int d = 1; d < kWave; d <<= 1) {
+        float pv[7];
 #pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) {
+        for (int j = 0; j < 7; ++j) pv[j] = __shfl_up(v[j], d, kWave);
+        const bool pf = __shfl_up((int)f, d, kWave) != 0;
+        if (lane >= d && !f) {
 #pragma unroll
-            for (int j = 0; j < 7; ++j) acc[j] += __shfl_xor(acc[j], o, kWave);
-        }
-        if (lane < 7) {
-            float v = acc[0];
-#pragma unroll
-            for (int j = 1; j < 7; ++j) v = lane == j ? acc[j] : v;
-            gq[7 * (int64_t)hk + lane] += v;
+            for (int j = 0; j < 7; ++j) v[j] += pv[j];
+            f = pf;
         }
     }
+    const uint32_t k0 = __shfl(k, 0, kWave), k63 = __shfl(k, kWave - 1, kWave);
+    const bool cont_in = p0 > 0 && key[p0 - 1] == k0;
+    const bool cont_out = p0 + kWave < n && key[p0 + kWave] == k63;
+    if (last && k != invalid) {
+        // the segment holds lane 0 when no segment starts in lanes 1..lane
+        const bool has0 = (starts & ((2ull << lane) - 2ull)) == 0;
+        float *dst;
+        if (has0 && cont_in) dst = head + 7 * w;
+        else if (lane == kWave - 1 && cont_out) dst = tail + 7 * w;
+        else dst = nullptr;
+        if (dst != nullptr) {
+#pragma unroll
+            for (int j = 0; j < 7; ++j) dst[j] = v[j];
+        } else {
+            float *g = gq + 7 * (int64_t)k;
+#pragma unroll
+            for (int j = 0; j < 7; ++j) g[j] += v[j];
+        }
+    }
+}
+
+// Pass 2: one thread per wave of pass 1.
+__global__ void __launch_bounds__(256) quadric_seg_join_kernel(const uint32_t *__restrict__ key,
+                                                               int64_t n, uint32_t invalid,
+                                                               const float *__restrict__ head,
+                                                               const float *__restrict__ tail,
+                                                               float *__restrict__ gq) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t p0 = w * kWave;
+    if (p0 + kWave >= n) return;  // the last wave has no run leaving it
+    const uint32_t k = key[p0 + kWave - 1];
+    if (k == invalid || key[p0 + kWave] != k) return;
+    if (p0 > 0 && key[p0] == k && key[p0 - 1] == k) return;  // a middle piece of an earlier run
+    float t[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) t[j] = tail[7 * w + j];
+    for (int64_t u = w + 1;; ++u) {
+#pragma unroll
+        for (int j = 0; j < 7; ++j) t[j] += head[7 * u + j];
+        const int64_t q = (u + 1) * kWave;
+        if (q >= n || key[q] != k) break;
+    }
+    float *g = gq + 7 * (int64_t)k;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) g[j] += t[j];
+}
+
+// Both passes over n sorted records; head and tail hold 7 floats per wave.
+template <class Src>
+int quadric_seg_sum(const uint32_t *key, Src src, int64_t n, uint32_t invalid, float *gq,
+                    float *head, float *tail, hipStream_t s) {
+    if (n == 0) return QSDVR_OK;
+    quadric_seg_kernel<<<grid_1d(n, 256), 256, 0, s>>>(key, src, n, invalid, gq, head, tail);
+    QSDVR_LAUNCHED();
+    const int64_t waves = (n + kWave - 1) / kWave;
+    quadric_seg_join_kernel<<<grid_1d(waves, 256), 256, 0, s>>>(key, n, invalid, head, tail, gq);
+    QSDVR_LAUNCHED();
+    return QSDVR_OK;
 }
 
 // Class boundaries in the sorted appearance keys: bounds[c] = first
@@ -320,7 +393,7 @@ __global__ void __launch_bounds__(256) det_app_reduce_kernel(
                 const int64_t start = p0 + my;
                 const int64_t end = later != 0 ? p0 + (__ffsll((long long)later) - 1) : tail_end;
                 float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 2
+#pragma unroll 4
                 for (int64_t i = start; i < end; ++i) {
                     const uint32_t r = idx[i];
                     const float g = ch < C ? src.grad(r, ch) : 0.0f;
@@ -353,9 +426,9 @@ int det_reduce(DetBuffers &b, int64_t T, float *gq, float *gdata, hipStream_t s)
     tb = b.temp_bytes;
     QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.akey, b.akey_s, b.aidx, b.aidx_s, (size_t)T,
                                         0u, (unsigned)(b.cbits + 4), s));
-    det_quadric_reduce_kernel<<<grid_1d(2 * T, 256), 256, 0, s>>>(
-        b.qkey_s, b.qidx_s, QuadricRecords{b.qval}, 2 * T, (uint32_t)R3, gq);
-    QSDVR_LAUNCHED();
+    int st = quadric_seg_sum(b.qkey_s, QuadricRecords{b.qval, b.qidx_s}, 2 * T, (uint32_t)R3, gq,
+                             b.qhead, b.qtail, s);
+    if (st != QSDVR_OK) return st;
     det_class_bounds_kernel<<<1, 64, 0, s>>>(b.akey_s, T, b.cbits, b.bounds);
     QSDVR_LAUNCHED();
     // each class holds about T/8 records; the waves stride over its range
@@ -377,8 +450,9 @@ int det_reduce(DetBuffers &b, int64_t T, float *gq, float *gdata, hipStream_t s)
 
 // ------------------------------------- op-surface sorted scatters ----
 // GridInterpolationBackward, SampleSDFBackward and SampleNormalBackward as
-// sorted-segment sums: the ops' own point lists, indices and upstream
-// gradients are the records; only the sample indices are sorted by key.
+// sorted-segment sums.  GridInterpolationBackward and SampleNormalBackward
+// sort sample indices by key and gather the records in sorted order;
+// SampleSDFBackward sorts its 16-B records (point, upstream) with the keys.
 
 // key of sample i: (parity class << cbits) | cell, invalid 8 << cbits
 __global__ void __launch_bounds__(256) op_interp_keys_kernel(const int32_t *__restrict__ idx,
@@ -398,11 +472,28 @@ __global__ void __launch_bounds__(256) op_interp_keys_kernel(const int32_t *__re
     val[i] = (uint32_t)i;
 }
 
-// key of sample i: its cell, invalid `cells`
-__global__ void __launch_bounds__(256) op_cell_keys_kernel(const int32_t *__restrict__ idx,
-                                                           int64_t n, int64_t cells,
-                                                           uint32_t *__restrict__ key,
-                                                           uint32_t *__restrict__ val) {
+// key of sample i: its cell, invalid `cells`; the record (point and
+// upstream gradient) is packed next to it and sorted with it, so that the
+// run sums read their records in order instead of gathering them
+__global__ void __launch_bounds__(256) op_sdf_keys_kernel(const int32_t *__restrict__ idx,
+                                                          const float *__restrict__ pts,
+                                                          const float *__restrict__ up,
+                                                          int64_t n, int64_t cells,
+                                                          uint32_t *__restrict__ key,
+                                                          float4 *__restrict__ pay) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t cell = idx[i];
+    key[i] = (cell >= 0 && cell < cells) ? (uint32_t)cell : (uint32_t)cells;
+    const f3 p = load3(pts, i);
+    pay[i] = float4{p.x, p.y, p.z, up[i]};
+}
+
+// key of sample i: its cell, invalid `cells`; value: i
+__global__ void __launch_bounds__(256) op_cell_idx_kernel(const int32_t *__restrict__ idx,
+                                                          int64_t n, int64_t cells,
+                                                          uint32_t *__restrict__ key,
+                                                          uint32_t *__restrict__ val) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t cell = idx[i];
@@ -412,11 +503,23 @@ __global__ void __launch_bounds__(256) op_cell_keys_kernel(const int32_t *__rest
 
 namespace {
 
+// val / val_s hold sample indices or SampleSDF's 16-B records
 struct OpBuffers {
-    uint32_t *key, *key_s, *val, *val_s, *bounds;
+    uint32_t *key, *key_s, *bounds;
+    char *val, *val_s;
+    float *head, *tail;  // quadric segment sums: 7 floats per wave of 64 records
     void *temp;
     size_t temp_bytes;
 };
+
+template <typename V>
+size_t sort_temp_bytes_v(int64_t n, int end_bit) {
+    size_t t = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, t, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                    (const V *)nullptr, (V *)nullptr, (size_t)n, 0u,
+                                    (unsigned)end_bit, (hipStream_t)0);
+    return t;
+}
 
 // smallest b with 2^b >= n (cell bits of the interpolation keys)
 int bits_at_least(uint64_t n) {
@@ -426,17 +529,20 @@ int bits_at_least(uint64_t n) {
 }
 
 size_t op_temp_bytes(int64_t n, int64_t cells) {
-    return std::max(sort_temp_bytes(n, bits_at_least((uint64_t)cells) + 4),
-                    sort_temp_bytes(n, bits_above((uint64_t)cells)));
+    const int qb = bits_above((uint64_t)cells);
+    return std::max({sort_temp_bytes(n, bits_at_least((uint64_t)cells) + 4),
+                     sort_temp_bytes(n, qb), sort_temp_bytes_v<float4>(n, qb)});
 }
 
 size_t op_carve(void *ws, size_t bytes, int64_t n, int64_t cells, OpBuffers &b) {
     Carver c{static_cast<char *>(ws), bytes};
     b.key = c.take<uint32_t>(n);
     b.key_s = c.take<uint32_t>(n);
-    b.val = c.take<uint32_t>(n);
-    b.val_s = c.take<uint32_t>(n);
+    b.val = c.take<char>(n * sizeof(float4));
+    b.val_s = c.take<char>(n * sizeof(float4));
     b.bounds = c.take<uint32_t>(16);
+    b.head = c.take<float>(7 * seg_waves(n));
+    b.tail = c.take<float>(7 * seg_waves(n));
     b.temp_bytes = op_temp_bytes(n, cells);
     b.temp = c.take<char>(b.temp_bytes);
     return c.used + 256;
@@ -492,10 +598,11 @@ int qsdvr_grid_interpolation_backward_sorted(float *grad_grid, const float *poin
     if (st != QSDVR_OK) return st;
     hipStream_t s = as_stream(stream);
     const int cbits = bits_at_least((uint64_t)cells);
-    op_interp_keys_kernel<<<grid_1d(n, 256), 256, 0, s>>>(index, n, R, cbits, b.key, b.val);
+    uint32_t *val = reinterpret_cast<uint32_t *>(b.val), *val_s = reinterpret_cast<uint32_t *>(b.val_s);
+    op_interp_keys_kernel<<<grid_1d(n, 256), 256, 0, s>>>(index, n, R, cbits, b.key, val);
     QSDVR_LAUNCHED();
     size_t tb = b.temp_bytes;
-    QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.key, b.key_s, b.val, b.val_s, (size_t)n, 0u,
+    QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.key, b.key_s, val, val_s, (size_t)n, 0u,
                                         (unsigned)(cbits + 4), s));
     det_class_bounds_kernel<<<1, 64, 0, s>>>(b.key_s, n, cbits, b.bounds);
     QSDVR_LAUNCHED();
@@ -505,26 +612,14 @@ int qsdvr_grid_interpolation_backward_sorted(float *grad_grid, const float *poin
     for (int cls = 0; cls < 8; ++cls) {
         if (channels == 32)
             det_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(
-                b.key_s, b.val_s, InterpSamples<32>{point, grad_out}, b.bounds, cls, cbits, R,
+                b.key_s, val_s, InterpSamples<32>{point, grad_out}, b.bounds, cls, cbits, R,
                 grad_grid);
         else
             det_app_reduce_kernel<53><<<blocks, 256, 0, s>>>(
-                b.key_s, b.val_s, InterpSamples<53>{point, grad_out}, b.bounds, cls, cbits, R,
+                b.key_s, val_s, InterpSamples<53>{point, grad_out}, b.bounds, cls, cbits, R,
                 grad_grid);
         QSDVR_LAUNCHED();
     }
-    return QSDVR_OK;
-}
-
-static int quadric_sorted(const int32_t *index, int64_t n, int64_t cells, void *workspace,
-                          size_t workspace_bytes, const char *who, hipStream_t s, OpBuffers &b) {
-    int st = op_prepare(workspace, workspace_bytes, n, cells, b, who);
-    if (st != QSDVR_OK) return st;
-    op_cell_keys_kernel<<<grid_1d(n, 256), 256, 0, s>>>(index, n, cells, b.key, b.val);
-    QSDVR_LAUNCHED();
-    size_t tb = b.temp_bytes;
-    QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.key, b.key_s, b.val, b.val_s, (size_t)n, 0u,
-                                        (unsigned)bits_above((uint64_t)cells), s));
     return QSDVR_OK;
 }
 
@@ -539,14 +634,18 @@ int qsdvr_sample_sdf_backward_sorted(const float *grad_out, const float *sdf_gri
                   "qsdvr_sample_sdf_backward_sorted: null pointer");
     hipStream_t s = as_stream(stream);
     OpBuffers b;
-    int st = quadric_sorted(index, n, cell_count, workspace, workspace_bytes,
-                            "qsdvr_sample_sdf_backward_sorted", s, b);
+    int st = op_prepare(workspace, workspace_bytes, n, cell_count, b,
+                        "qsdvr_sample_sdf_backward_sorted");
     if (st != QSDVR_OK) return st;
-    det_quadric_reduce_kernel<<<grid_1d(n, 256), 256, 0, s>>>(
-        b.key_s, b.val_s, SdfSamples{sdf_grid, point, grad_out, 1.0f / reso}, n,
-        (uint32_t)cell_count, grad_grid);
+    float4 *pay = reinterpret_cast<float4 *>(b.val), *pay_s = reinterpret_cast<float4 *>(b.val_s);
+    op_sdf_keys_kernel<<<grid_1d(n, 256), 256, 0, s>>>(index, point, grad_out, n, cell_count,
+                                                       b.key, pay);
     QSDVR_LAUNCHED();
-    return QSDVR_OK;
+    size_t tb = b.temp_bytes;
+    QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.key, b.key_s, pay, pay_s, (size_t)n, 0u,
+                                        (unsigned)bits_above((uint64_t)cell_count), s));
+    return quadric_seg_sum(b.key_s, SdfSamples{pay_s, sdf_grid, 1.0f / reso}, n,
+                           (uint32_t)cell_count, grad_grid, b.head, b.tail, s);
 }
 
 int qsdvr_sample_normal_backward_sorted(const float *grad_out, const float *sdf_grid,
@@ -560,14 +659,17 @@ int qsdvr_sample_normal_backward_sorted(const float *grad_out, const float *sdf_
                   "qsdvr_sample_normal_backward_sorted: null pointer");
     hipStream_t s = as_stream(stream);
     OpBuffers b;
-    int st = quadric_sorted(index, n, cell_count, workspace, workspace_bytes,
-                            "qsdvr_sample_normal_backward_sorted", s, b);
+    int st = op_prepare(workspace, workspace_bytes, n, cell_count, b,
+                        "qsdvr_sample_normal_backward_sorted");
     if (st != QSDVR_OK) return st;
-    det_quadric_reduce_kernel<<<grid_1d(n, 256), 256, 0, s>>>(
-        b.key_s, b.val_s, NormalSamples{sdf_grid, point, grad_out}, n, (uint32_t)cell_count,
-        grad_grid);
+    uint32_t *val = reinterpret_cast<uint32_t *>(b.val), *val_s = reinterpret_cast<uint32_t *>(b.val_s);
+    op_cell_idx_kernel<<<grid_1d(n, 256), 256, 0, s>>>(index, n, cell_count, b.key, val);
     QSDVR_LAUNCHED();
-    return QSDVR_OK;
+    size_t tb = b.temp_bytes;
+    QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.key, b.key_s, val, val_s, (size_t)n, 0u,
+                                        (unsigned)bits_above((uint64_t)cell_count), s));
+    return quadric_seg_sum(b.key_s, NormalSamples{val_s, sdf_grid, point, grad_out}, n,
+                           (uint32_t)cell_count, grad_grid, b.head, b.tail, s);
 }
 
 }  // extern "C"
