@@ -40,6 +40,47 @@ __global__ void __launch_bounds__(256) interp_fwd_kernel(const float *__restrict
     out[id] = lerp_ref(a0, a1, p.z);
 }
 
+// C = 32: lane = (sample slot, channel quad), 8 lanes per sample with one
+// 16-B load per vertex row, 8 samples per wave step and ITER steps per wave
+// (the loads of all steps in flight together); the same lerp order per
+// channel as interp_fwd_kernel.
+__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float w) {
+    return float4{lerp_ref(a.x, b.x, w), lerp_ref(a.y, b.y, w), lerp_ref(a.z, b.z, w),
+                  lerp_ref(a.w, b.w, w)};
+}
+
+template <int ITER>
+__global__ void __launch_bounds__(256) interp_fwd32_kernel(const float4 *__restrict__ data,
+                                                           const float *__restrict__ pts,
+                                                           const int32_t *__restrict__ idx,
+                                                           float4 *__restrict__ out, int64_t n,
+                                                           int R) {
+    const int lane = threadIdx.x % kWave, q = lane & 7, slot = lane >> 3;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t cells = (int64_t)R * R * R, RR = (int64_t)R * R;
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+        const int64_t s = (wave * ITER + it) * 8 + slot;
+        if (s >= n) continue;
+        const int64_t cell = idx[s];
+        if (cell < 0 || cell >= cells) {
+            const float nan = __builtin_nanf("");
+            out[s * 8 + q] = float4{nan, nan, nan, nan};
+            continue;
+        }
+        const f3 p = load3(pts, s);
+        const float4 *b = data + cell * 8 + q;
+        auto D = [&](int64_t off) { return b[off * 8]; };
+        const float4 a00 = lerp4(D(0), D(1), p.x);
+        const float4 a01 = lerp4(D(R), D(R + 1), p.x);
+        const float4 a0 = lerp4(a00, a01, p.y);
+        const float4 a10 = lerp4(D(RR), D(RR + 1), p.x);
+        const float4 a11 = lerp4(D(RR + R), D(RR + R + 1), p.x);
+        const float4 a1 = lerp4(a10, a11, p.y);
+        out[s * 8 + q] = lerp4(a0, a1, p.z);
+    }
+}
+
 template <int CFIX>
 __global__ void __launch_bounds__(256) interp_bwd_kernel(float *__restrict__ gdata,
                                                          const float *__restrict__ pts,
@@ -252,7 +293,14 @@ int qsdvr_grid_interpolation_forward(const float *data_grid, const float *point,
     int R = vertex_reso - 1;
     int64_t threads = n * channels;
     hipStream_t s = as_stream(stream);
-    if (channels == 32)
+    const bool aligned16 = ((uintptr_t)data_grid % 16 == 0) && ((uintptr_t)out % 16 == 0);
+    if (channels == 32 && aligned16) {
+        constexpr int kIter = 4;  // samples per wave: 8 kIter
+        const int64_t waves = (n + 8 * kIter - 1) / (8 * kIter);
+        interp_fwd32_kernel<kIter><<<grid_1d(waves * kWave, 256), 256, 0, s>>>(
+            reinterpret_cast<const float4 *>(data_grid), point, index, reinterpret_cast<float4 *>(out),
+            n, R);
+    } else if (channels == 32)
         interp_fwd_kernel<32><<<grid_1d(threads, 256), 256, 0, s>>>(data_grid, point, index, out,
                                                                     n, R, 32);
     else
