@@ -14,7 +14,12 @@
 // ray front to back again, rebuilding e, alpha, T and the prefix
 // sum_{i<=k} w_i rgb_i; the suffix the alpha adjoint needs is
 // saved_colour - prefix, and 1/(1 - alpha_k) = (1 + e_{k+1}) / (1 + e_k)
-// exactly (no division by a vanishing 1 - alpha).
+// exactly (no division by a vanishing 1 - alpha).  d alpha_j / d sdf_{j+1} =
+// -s e_{j+1} (1 + e_j) / (1 + e_{j+1})^2 is evaluated as the product of
+// e_{j+1} / (1 + e_{j+1}) and (1 + e_j) / (1 + e_{j+1}), both at most 1
+// where alpha is not clamped: e_{j+1} (1 + e_j) alone overflows float32
+// where the SDF is below about -1.1 / s (s = 40: non-metric SDF values of
+// the gyroid shell reach that).
 #include "det_scatter.h"
 #include "quadric.h"
 #include "raygen.h"
@@ -520,7 +525,7 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_kernel(
                            gD * (Ds - pre_d);
             float da = T * here - suffix * one_over;
             g_sdf0 += da * s * e0 * inv1;
-            g_sdf1 = -da * s * e1 * (1.0f + e0) * inv1 * inv1;
+            g_sdf1 = -da * s * (e1 * inv1) * ((1.0f + e0) * inv1);
         }
         // sdf sample j is final: scatter its quadric gradient
         {
@@ -639,7 +644,7 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
                            gD * (Ds - pre_d);
             float da = T * here - suffix * one_over;
             g_sdf0 += da * s * e0 * inv1;
-            g_sdf1 = -da * s * e1 * (1.0f + e0) * inv1 * inv1;
+            g_sdf1 = -da * s * (e1 * inv1) * ((1.0f + e0) * inv1);
         }
         {
             float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -995,7 +1000,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                 float here = gC.x * ev.rgb.x + gC.y * ev.rgb.y + gC.z * ev.rgb.z + gD * ev.t;
                 float da = T * here - suffix * one_over;
                 g_sdf0 += da * s * e0 * inv1;
-                g_sdf1 = -da * s * e1 * (1.0f + e0) * inv1 * inv1;
+                g_sdf1 = -da * s * (e1 * inv1) * ((1.0f + e0) * inv1);
                 if (wt != 0.0f) {
                     f3 grgb{gC.x * wt, gC.y * wt, gC.z * wt};
                     f3 gn = shade_vjp<NC>(ev.normal, ray.dir, d, ev.spec, grgb, dd);
@@ -1013,7 +1018,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                                gC.z * (Cs.z - pre.z) + gD * (Ds - pre_d);
                 float da = -suffix * one_over;
                 g_sdf0 += da * s * e0 * inv1;
-                g_sdf1 = -da * s * e1 * (1.0f + e0) * inv1 * inv1;
+                g_sdf1 = -da * s * (e1 * inv1) * ((1.0f + e0) * inv1);
             }
             T = T * (1.0f - a);
             e0 = e1;

@@ -29,8 +29,12 @@ __device__ __forceinline__ float quadric_sdf(const Quadric &q, f3 p, float inv_r
 // Adds d sdf / d(A..G) * upstream to g[7] (closed form of the quotient rule):
 // k = 1/||grad f||, q = f k^3, s = upstream / reso;
 // dA = s x (x k - 2 g_x q), dD = s (x k - g_x q), ..., dG = s k.
+// A zero upstream adds nothing, also where grad f vanishes (k infinite):
+// the renderers give such a sample alpha 0, so its upstream is 0, and
+// 0 * inf must not turn the cell's gradient into NaN.
 __device__ __forceinline__ void quadric_sdf_grad(const Quadric &q, f3 p, float upstream,
                                                  float inv_reso, float g[7]) {
+    if (upstream == 0.0f) return;
     float a0 = q.a * p.x, a1 = q.b * p.y, a2 = q.c * p.z;
     float f = (a0 + q.d) * p.x + (a1 + q.e) * p.y + (a2 + q.f) * p.z + q.g;
     float gx = 2.0f * a0 + q.d, gy = 2.0f * a1 + q.e, gz = 2.0f * a2 + q.f;

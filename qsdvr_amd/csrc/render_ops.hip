@@ -261,8 +261,12 @@ __global__ void __launch_bounds__(256) aggregate_bwd_kernel(
         const float inv = 1.0f / (1.0f + e1);
         const float pre = (e1 - e0) * inv;
         const float pass = (valid && pre >= 0.0f && pre <= 10.0f) ? da : 0.0f;
-        const float d_s0 = pass * s * e0 * inv;                        // d alpha_k / d sdf_k
-        const float d_s1 = -pass * s * e1 * (1.0f + e0) * inv * inv;   // d alpha_k / d sdf_{k+1}
+        // d alpha_k / d sdf_k and d alpha_k / d sdf_{k+1}; where the clamp
+        // passes, e0 <= e1 < inf, so every factor below is at most 1 (the
+        // product e1 (1 + e0) alone would overflow), and a clamped or
+        // non-finite alpha contributes exactly 0
+        const float d_s0 = pass != 0.0f ? pass * s * e0 * inv : 0.0f;
+        const float d_s1 = pass != 0.0f ? -pass * s * (e1 * inv) * ((1.0f + e0) * inv) : 0.0f;
         float nxt = __shfl_down(d_s0, 1, kWave);
         if (lane == kWave - 1) nxt = ds0_next;
         if (valid) {

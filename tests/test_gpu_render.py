@@ -110,3 +110,32 @@ def test_fused_ray_list_matches_full_frame(small):
     c_sub, d_sub = fused.render(G, D, [cam], w.interval, w.s, ray_pixel=pick.cuda())
     assert torch.equal(c_sub.cpu(), c_full.cpu()[pick.long()])
     assert torch.equal(d_sub.cpu(), d_full.cpu()[pick.long()])
+
+
+def test_degenerate_cells_keep_gradients_finite(small):
+    """A cell whose quadric has no gradient (A..F = 0) has an infinite SDF
+    (f / |grad f| with |grad f| = 0).  The renderers give its sdf samples
+    alpha 0, so their upstream gradient is 0, and a zero upstream must add
+    nothing rather than 0 * inf = NaN: every gradient stays finite, in every
+    scatter mode and on the op surface."""
+    w, cam, grid, data, target, _ = small
+    g = grid.clone()
+    g[w.R // 2 - 1:w.R // 2 + 1, :, :, :6] = 0.0  # a slab of degenerate cells across the object
+    g[..., 6] = torch.where(g[..., 6] == 0, torch.full_like(g[..., 6], 0.5), g[..., 6])
+    tgt = target.cuda()
+    for mode in (fused.SCATTER_TILE, fused.SCATTER_ATOMIC, fused.SCATTER_DET):
+        G = g.cuda().requires_grad_(True)
+        D = data.cuda().requires_grad_(True)
+        c, d = fused.render(G, D, [cam], w.interval, w.s, scatter_mode=mode)
+        ((c - tgt) ** 2).mean().backward()
+        assert torch.isfinite(c).all() and torch.isfinite(d).all(), mode
+        assert torch.isfinite(G.grad).all() and torch.isfinite(D.grad).all(), mode
+    a = qsdvr.CameraArgs.from_camera(cam)
+    cc = Camera(a, w.interval, w.R)
+    cc.GenerateRayPoints(None)
+    G = g.cuda().requires_grad_(True)
+    D = data.cuda().requires_grad_(True)
+    out = hr.render_lists(G, D, cc.GetRayPoints(), w.s)
+    ((out - tgt) ** 2).mean().backward()
+    assert torch.isfinite(out).all()
+    assert torch.isfinite(G.grad).all() and torch.isfinite(D.grad).all()
