@@ -371,6 +371,15 @@ __device__ __forceinline__ void eval_render_sample(const RaySetup &ray, uint32_t
     e.t = fast_sqrt(px * px + py * py + pz * pz);
 }
 
+// Cell point of sdf sample j (as sdf_at computes it).
+__device__ __forceinline__ CellPoint ray_sdf_point(const RaySetup &ray, uint32_t j, float interval,
+                                                   float half_reso, int R) {
+    CellPoint cp = ray.tl ? point_at(ray, ray.tl[j], half_reso, R)
+                          : sdf_sample(ray.ori, ray.dir, j, interval, half_reso, R);
+    clamp_cell(cp, R);
+    return cp;
+}
+
 __device__ __forceinline__ float sdf_at(const RaySetup &ray, uint32_t j, float interval,
                                         float half_reso, int R, float inv_reso,
                                         const float *__restrict__ qgrid, CellPoint &cp) {
@@ -461,6 +470,121 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
     }
     store3(color, ray.out, col);
     if (depth) depth[ray.out] = dep;
+}
+
+// ---------------------------------------------------------------------------
+// Listed rays (random batches over many poses): one wavefront per ray, lanes
+// over its samples, 64 at a time.  A lane-per-ray walk leaves too few
+// waves to hide the per-sample gather latency at batch sizes like 65,536
+// rays (1,024 waves); here every sample is a lane.  T_j is the carried
+// transmittance times an exclusive product scan of (1 - alpha) over the
+// lanes.  Results agree with the lane-per-ray walk to float32 rounding (the
+// products and sums are formed in tree order).
+
+// Ray i of the list, set up by every lane of the wave (RayCut in parallel
+// over the candidates, or the precomputed ray state).
+__device__ __forceinline__ RaySetup setup_ray_wave(const qsdvr_camera *__restrict__ cams,
+                                                   int n_cams, uint32_t W, uint32_t H,
+                                                   const int32_t *__restrict__ ray_pixel,
+                                                   int64_t i, float interval,
+                                                   const float4 *__restrict__ ray_state) {
+    RaySetup r;
+    r.out = -1;
+    r.renders = 0;
+    r.tl = nullptr;
+    const int64_t code = (uint32_t)ray_pixel[i];
+    const int cam = (int)(code / ((int64_t)W * H));
+    if (cam >= n_cams) return r;
+    const uint32_t pix = (uint32_t)(code - (int64_t)cam * W * H);
+    const qsdvr_camera &c = cams[cam];
+    if (c.width != W || c.height != H) return r;
+    r.out = i;
+    r.cam = cam;
+    r.center = f3{c.translation[0], c.translation[1], c.translation[2]};
+    uint32_t count;
+    if (ray_state != nullptr) {
+        const float4 a = ray_state[2 * i], b = ray_state[2 * i + 1];
+        r.ori = f3{a.x, a.y, a.z};
+        r.dir = f3{b.x, b.y, b.z};
+        count = __float_as_uint(a.w);
+    } else {
+        const RayCutResult cut = ray_cut_wave(c, pix, interval);
+        r.ori = cut.origin;
+        r.dir = cut.dir;
+        count = cut.count;
+    }
+    r.renders = count >= 2 ? count - 1 : 0;
+    return r;
+}
+
+// Exclusive product over lanes < lane of x (all lanes call it).
+__device__ __forceinline__ float wave_excl_product(float x, float &total) {
+    const int lane = threadIdx.x % kWave;
+    float p = x;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const float u = __shfl_up(p, o, kWave);
+        if (lane >= o) p *= u;
+    }
+    total = __shfl(p, kWave - 1, kWave);
+    const float up = __shfl_up(p, 1, kWave);
+    return lane == 0 ? 1.0f : up;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kFusedBlock) fused_fwd_ray_kernel(
+    const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
+    const int32_t *__restrict__ ray_pixel, int64_t n_rays, const float *__restrict__ qgrid,
+    const float *__restrict__ data, int R, float interval, float s, float *__restrict__ color,
+    float *__restrict__ depth) {
+    constexpr int C = 3 * NC + 5;
+    const int lane = threadIdx.x % kWave;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    if (i >= n_rays) return;  // wave-uniform
+    const RaySetup ray = setup_ray_wave(cams, n_cams, W, H, ray_pixel, i, interval, nullptr);
+    if (ray.out < 0) return;
+    f3 col{0.0f, 0.0f, 0.0f};
+    float dep = 0.0f;
+    if (ray.renders > 0) {
+        const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
+        CellPoint cp;
+        float e_prev = expf(-s * sdf_at(ray, 0, interval, half_reso, R, inv_reso, qgrid, cp));
+        float T = 1.0f;
+        for (uint32_t c0 = 0; c0 < ray.renders && T != 0.0f; c0 += kWave) {
+            const uint32_t j = c0 + (uint32_t)lane;
+            const bool act = j < ray.renders;
+            float e1 = 0.0f;
+            if (act) e1 = expf(-s * sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp));
+            const float e_up = __shfl_up(e1, 1, kWave);
+            const float e0 = lane == 0 ? e_prev : e_up;
+            const float a = act ? fmaxf((e1 - e0) * fast_rcp(1.0f + e1), 0.0f) : 0.0f;
+            float chunk;
+            const float wt = a * (T * wave_excl_product(1.0f - a, chunk));
+            if (wt != 0.0f) {
+                float w[8], d[C];
+                SampleEval ev;
+                eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
+                col.x += ev.rgb.x * wt;
+                col.y += ev.rgb.y * wt;
+                col.z += ev.rgb.z * wt;
+                dep += ev.t * wt;
+            }
+            T *= chunk;
+            e_prev = __shfl(e1, kWave - 1, kWave);
+        }
+        col = f3{wave_sum(col.x), wave_sum(col.y), wave_sum(col.z)};
+        dep = wave_sum(dep);
+    }
+    if (lane == 0) {
+        store3(color, ray.out, col);
+        if (depth) depth[ray.out] = dep;
+    }
 }
 
 __device__ __forceinline__ void scatter_quadric(float *__restrict__ g, int32_t cell,
@@ -692,6 +816,147 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
         float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
         quadric_sdf_grad(load_quadric(qgrid, cp0.cell), cp0.local, carry, inv_reso, gq7);
         det_put_quadric(o, 2 * q, (uint32_t)cp0.cell, gq7);
+        const float zero7[7] = {0, 0, 0, 0, 0, 0, 0};
+        det_put_quadric(o, 2 * q + 1, o.q_invalid, zero7);
+        o.akey[q] = 8u << o.cbits;
+        o.aidx[q] = q;
+    }
+}
+
+// fused_bwd_record_kernel for listed rays with uniform samples, one
+// wavefront per ray as fused_fwd_ray_kernel: lane l of chunk c0 takes
+// render sample j = c0 + l and finalises sdf sample j.  T is the carried
+// transmittance times an exclusive product scan of (1 - alpha); the prefix
+// sum_{i<=j} w_i rgb_i an inclusive sum scan; sdf sample j's gradient is
+// its own alpha_j term plus lane j-1's alpha_{j-1} term.  The records are
+// those of the lane-per-ray kernel, in the same slots.
+__device__ __forceinline__ float wave_incl_sum(float x) {
+    const int lane = threadIdx.x % kWave;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const float u = __shfl_up(x, o, kWave);
+        if (lane >= o) x += u;
+    }
+    return x;
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_ray_kernel(
+    const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
+    const int32_t *__restrict__ ray_pixel, int64_t n_rays, const float *__restrict__ qgrid,
+    const float *__restrict__ data, int R, float interval, float s,
+    const float *__restrict__ color, const float *__restrict__ depth,
+    const float *__restrict__ gcolor, const float *__restrict__ gdepth,
+    const float4 *__restrict__ ray_state, DetRecordOut o) {
+    constexpr int C = 3 * NC + 5;
+    constexpr int AP = 4 + (C + 3) / 4 * 4;
+    const int lane = threadIdx.x % kWave;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    if (i >= n_rays) return;  // wave-uniform
+    const RaySetup ray = setup_ray_wave(cams, n_cams, W, H, ray_pixel, i, interval, ray_state);
+    if (ray.out < 0 || ray.renders == 0) return;
+    const uint32_t base = o.offsets[ray.out];
+    const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
+    const f3 Cs = load3(color, ray.out);
+    const f3 gC = load3(gcolor, ray.out);
+    const float Ds = depth ? depth[ray.out] : 0.0f;
+    const float gD = gdepth ? gdepth[ray.out] : 0.0f;
+    CellPoint cp;
+    float e_prev = expf(-s * sdf_at(ray, 0, interval, half_reso, R, inv_reso, qgrid, cp));
+    float T = 1.0f, g1_prev = 0.0f;
+    f3 pre{0.0f, 0.0f, 0.0f};
+    float pre_d = 0.0f;
+    for (uint32_t c0 = 0; c0 < ray.renders; c0 += kWave) {
+        const uint32_t j = c0 + (uint32_t)lane;
+        const bool act = j < ray.renders;
+        const uint32_t q = base + j;
+        CellPoint cp0, cp1;
+        cp0.cell = 0;
+        float e1 = 0.0f;
+        if (act) {
+            cp0 = ray_sdf_point(ray, j, interval, half_reso, R);
+            e1 = expf(-s * sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp1));
+        }
+        const float e_up = __shfl_up(e1, 1, kWave);
+        const float e0 = lane == 0 ? e_prev : e_up;
+        const float inv1 = fast_rcp(1.0f + e1);
+        const float praw = act ? (e1 - e0) * inv1 : -1.0f;
+        const float a = fmaxf(praw, 0.0f);
+        float chunk;
+        const float Tj = T * wave_excl_product(1.0f - a, chunk);
+        const float wt = a * Tj;
+        const bool need = praw >= 0.0f && Tj != 0.0f;  // as in fused_bwd_record_kernel
+        float w[8], d[C];
+        SampleEval ev;
+        float here = 0.0f;
+        f3 rw{0.0f, 0.0f, 0.0f};
+        float tw = 0.0f;
+        if (need) {
+            eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
+            rw = f3{ev.rgb.x * wt, ev.rgb.y * wt, ev.rgb.z * wt};
+            tw = ev.t * wt;
+            here = gC.x * ev.rgb.x + gC.y * ev.rgb.y + gC.z * ev.rgb.z + gD * ev.t;
+        }
+        const f3 P{pre.x + wave_incl_sum(rw.x), pre.y + wave_incl_sum(rw.y),
+                   pre.z + wave_incl_sum(rw.z)};
+        const float Pd = pre_d + wave_incl_sum(tw);
+        float g0 = 0.0f, g1 = 0.0f;
+        if (praw >= 0.0f) {
+            const float one_over = (1.0f + e1) * fast_rcp(1.0f + e0);
+            const float suffix = gC.x * (Cs.x - P.x) + gC.y * (Cs.y - P.y) +
+                                 gC.z * (Cs.z - P.z) + gD * (Ds - Pd);
+            const float da = Tj * here - suffix * one_over;
+            g0 = da * s * e0 * inv1;
+            g1 = -da * s * (e1 * inv1) * ((1.0f + e0) * inv1);
+        }
+        const float g1_up = __shfl_up(g1, 1, kWave);
+        const float g_sdf = g0 + (lane == 0 ? g1_prev : g1_up);
+        if (act) {
+            float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
+            quadric_sdf_grad(load_quadric(qgrid, cp0.cell), cp0.local, g_sdf, inv_reso, gq7);
+            det_put_quadric(o, 2 * q, (uint32_t)cp0.cell, gq7);
+            if (!(need && wt != 0.0f)) {
+                const float zero7[7] = {0, 0, 0, 0, 0, 0, 0};
+                det_put_quadric(o, 2 * q + 1, o.q_invalid, zero7);
+                o.akey[q] = 8u << o.cbits;
+                o.aidx[q] = q;
+            } else {
+                const f3 grgb{gC.x * wt, gC.y * wt, gC.z * wt};
+                float dd[C];
+                const f3 gn = shade_vjp<NC>(ev.normal, ray.dir, d, ev.spec, grgb, dd);
+                float gq6[7] = {0, 0, 0, 0, 0, 0, 0};
+                quadric_normal_grad(load_quadric(qgrid, ev.rp.cell), ev.rp.local, gn, gq6);
+                det_put_quadric(o, 2 * q + 1, (uint32_t)ev.rp.cell, gq6);
+                const uint32_t cls = (uint32_t)((ev.rp.ix & 1) | ((ev.rp.iy & 1) << 1) |
+                                                ((ev.rp.iz & 1) << 2));
+                o.akey[q] = (cls << o.cbits) | (uint32_t)ev.rp.cell;
+                o.aidx[q] = q;
+                float4 *dst = reinterpret_cast<float4 *>(o.aval + (int64_t)q * AP);
+                dst[0] = make_float4(ev.rp.local.x, ev.rp.local.y, ev.rp.local.z, 0.0f);
+#pragma unroll
+                for (int c = 0; c < (AP - 4) / 4; ++c) {
+                    const int cc = 4 * c;
+                    dst[1 + c] = make_float4(dd[cc], cc + 1 < C ? dd[cc + 1] : 0.0f,
+                                             cc + 2 < C ? dd[cc + 2] : 0.0f,
+                                             cc + 3 < C ? dd[cc + 3] : 0.0f);
+                }
+            }
+        }
+        // carries into the next chunk: the last lane of this one (lane 63
+        // unless this is the ray's last chunk)
+        const int last = (int)min((uint32_t)(kWave - 1), ray.renders - 1 - c0);
+        T *= chunk;
+        e_prev = __shfl(e1, last, kWave);
+        g1_prev = __shfl(g1, last, kWave);
+        pre = f3{__shfl(P.x, last, kWave), __shfl(P.y, last, kWave), __shfl(P.z, last, kWave)};
+        pre_d = __shfl(Pd, last, kWave);
+    }
+    if (lane == 0) {  // the last sdf sample: only alpha_{renders-1}'s term
+        const uint32_t q = base + ray.renders;
+        const CellPoint cpl = ray_sdf_point(ray, ray.renders, interval, half_reso, R);
+        float gq7[7] = {0, 0, 0, 0, 0, 0, 0};
+        quadric_sdf_grad(load_quadric(qgrid, cpl.cell), cpl.local, g1_prev, inv_reso, gq7);
+        det_put_quadric(o, 2 * q, (uint32_t)cpl.cell, gq7);
         const float zero7[7] = {0, 0, 0, 0, 0, 0, 0};
         det_put_quadric(o, 2 * q + 1, o.q_invalid, zero7);
         o.akey[q] = 8u << o.cbits;
@@ -1330,7 +1595,18 @@ static int fused_forward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t 
     fused_fwd_kernel<NC, BINS><<<blocks, kFusedBlock, fwd_lds_bytes<NC>(), s>>>(             \
         cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid, data_grid, \
         reso, interval, logistic_coef, color, depth, rs, tlist, K, bins)
-    if (channels == 32) {
+    if (ray_pixel != nullptr && tlist == nullptr && bins == nullptr) {
+        // listed rays with uniform samples: one wavefront per ray
+        const unsigned rblocks = (unsigned)((n_rays + kWavesPerBlock - 1) / kWavesPerBlock);
+        if (channels == 32)
+            fused_fwd_ray_kernel<9><<<rblocks, kFusedBlock, 0, s>>>(
+                cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
+                data_grid, reso, interval, logistic_coef, color, depth);
+        else
+            fused_fwd_ray_kernel<16><<<rblocks, kFusedBlock, 0, s>>>(
+                cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
+                data_grid, reso, interval, logistic_coef, color, depth);
+    } else if (channels == 32) {
         if (bins) QSDVR_FWD(9, true);
         else QSDVR_FWD(9, false);
     } else {
@@ -1531,7 +1807,20 @@ int qsdvr_fused_backward_det(const qsdvr_camera *cams, int32_t n_cams, int32_t w
     const float *dsave = grad_depth ? depth : nullptr;
     const int64_t waves = fused_waves(width, height, n_cams, ray_pixel, n_rays);
     const unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
-    if (channels == 32)
+    if (ray_pixel != nullptr && tlist == nullptr) {
+        // listed rays with uniform samples: one wavefront per ray
+        const unsigned rblocks = (unsigned)((n_rays + kWavesPerBlock - 1) / kWavesPerBlock);
+        if (channels == 32)
+            fused_bwd_record_ray_kernel<9><<<rblocks, kFusedBlock, 0, s>>>(
+                cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
+                data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth, rs,
+                o);
+        else
+            fused_bwd_record_ray_kernel<16><<<rblocks, kFusedBlock, 0, s>>>(
+                cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
+                data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth, rs,
+                o);
+    } else if (channels == 32)
         fused_bwd_record_kernel<9><<<blocks, kFusedBlock, 0, s>>>(
             cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
             data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth, rs,
