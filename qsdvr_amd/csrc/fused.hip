@@ -473,16 +473,16 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Listed rays (random batches over many poses): one wavefront per ray, lanes
-// over its samples, 64 at a time.  A lane-per-ray walk leaves too few
-// waves to hide the per-sample gather latency at batch sizes like 65,536
-// rays (1,024 waves); here every sample is a lane.  T_j is the carried
-// transmittance times an exclusive product scan of (1 - alpha) over the
-// lanes.  Results agree with the lane-per-ray walk to float32 rounding (the
-// products and sums are formed in tree order).
+// Listed rays (random batches over many poses) in the deterministic adjoint:
+// one wavefront per ray, lanes over its samples, 64 at a time.  A
+// lane-per-ray walk leaves too few waves to hide the latency of the
+// per-sample adjoint at batch sizes like 65,536 rays (1,024 waves); here
+// every sample is a lane.  (The forward of such batches stays lane per ray:
+// its random row gathers already run at the memory system's rate, and the
+// wave form measured no faster.)
 
-// Ray i of the list, set up by every lane of the wave (RayCut in parallel
-// over the candidates, or the precomputed ray state).
+// Ray i of the list, set up by every lane of the wave from the precomputed
+// ray state (qsdvr_ray_state).
 __device__ __forceinline__ RaySetup setup_ray_wave(const qsdvr_camera *__restrict__ cams,
                                                    int n_cams, uint32_t W, uint32_t H,
                                                    const int32_t *__restrict__ ray_pixel,
@@ -501,18 +501,10 @@ __device__ __forceinline__ RaySetup setup_ray_wave(const qsdvr_camera *__restric
     r.out = i;
     r.cam = cam;
     r.center = f3{c.translation[0], c.translation[1], c.translation[2]};
-    uint32_t count;
-    if (ray_state != nullptr) {
-        const float4 a = ray_state[2 * i], b = ray_state[2 * i + 1];
-        r.ori = f3{a.x, a.y, a.z};
-        r.dir = f3{b.x, b.y, b.z};
-        count = __float_as_uint(a.w);
-    } else {
-        const RayCutResult cut = ray_cut_wave(c, pix, interval);
-        r.ori = cut.origin;
-        r.dir = cut.dir;
-        count = cut.count;
-    }
+    const float4 a = ray_state[2 * i], b = ray_state[2 * i + 1];
+    r.ori = f3{a.x, a.y, a.z};
+    r.dir = f3{b.x, b.y, b.z};
+    const uint32_t count = __float_as_uint(a.w);
     r.renders = count >= 2 ? count - 1 : 0;
     return r;
 }
@@ -529,62 +521,6 @@ __device__ __forceinline__ float wave_excl_product(float x, float &total) {
     total = __shfl(p, kWave - 1, kWave);
     const float up = __shfl_up(p, 1, kWave);
     return lane == 0 ? 1.0f : up;
-}
-
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
-}
-
-template <int NC>
-__global__ void __launch_bounds__(kFusedBlock) fused_fwd_ray_kernel(
-    const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
-    const int32_t *__restrict__ ray_pixel, int64_t n_rays, const float *__restrict__ qgrid,
-    const float *__restrict__ data, int R, float interval, float s, float *__restrict__ color,
-    float *__restrict__ depth) {
-    constexpr int C = 3 * NC + 5;
-    const int lane = threadIdx.x % kWave;
-    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    if (i >= n_rays) return;  // wave-uniform
-    const RaySetup ray = setup_ray_wave(cams, n_cams, W, H, ray_pixel, i, interval, nullptr);
-    if (ray.out < 0) return;
-    f3 col{0.0f, 0.0f, 0.0f};
-    float dep = 0.0f;
-    if (ray.renders > 0) {
-        const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
-        CellPoint cp;
-        float e_prev = expf(-s * sdf_at(ray, 0, interval, half_reso, R, inv_reso, qgrid, cp));
-        float T = 1.0f;
-        for (uint32_t c0 = 0; c0 < ray.renders && T != 0.0f; c0 += kWave) {
-            const uint32_t j = c0 + (uint32_t)lane;
-            const bool act = j < ray.renders;
-            float e1 = 0.0f;
-            if (act) e1 = expf(-s * sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp));
-            const float e_up = __shfl_up(e1, 1, kWave);
-            const float e0 = lane == 0 ? e_prev : e_up;
-            const float a = act ? fmaxf((e1 - e0) * fast_rcp(1.0f + e1), 0.0f) : 0.0f;
-            float chunk;
-            const float wt = a * (T * wave_excl_product(1.0f - a, chunk));
-            if (wt != 0.0f) {
-                float w[8], d[C];
-                SampleEval ev;
-                eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
-                col.x += ev.rgb.x * wt;
-                col.y += ev.rgb.y * wt;
-                col.z += ev.rgb.z * wt;
-                dep += ev.t * wt;
-            }
-            T *= chunk;
-            e_prev = __shfl(e1, kWave - 1, kWave);
-        }
-        col = f3{wave_sum(col.x), wave_sum(col.y), wave_sum(col.z)};
-        dep = wave_sum(dep);
-    }
-    if (lane == 0) {
-        store3(color, ray.out, col);
-        if (depth) depth[ray.out] = dep;
-    }
 }
 
 __device__ __forceinline__ void scatter_quadric(float *__restrict__ g, int32_t cell,
@@ -824,7 +760,7 @@ __global__ void __launch_bounds__(kFusedBlock) fused_bwd_record_kernel(
 }
 
 // fused_bwd_record_kernel for listed rays with uniform samples, one
-// wavefront per ray as fused_fwd_ray_kernel: lane l of chunk c0 takes
+// wavefront per ray: lane l of chunk c0 takes
 // render sample j = c0 + l and finalises sdf sample j.  T is the carried
 // transmittance times an exclusive product scan of (1 - alpha); the prefix
 // sum_{i<=j} w_i rgb_i an inclusive sum scan; sdf sample j's gradient is
@@ -1595,18 +1531,7 @@ static int fused_forward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t 
     fused_fwd_kernel<NC, BINS><<<blocks, kFusedBlock, fwd_lds_bytes<NC>(), s>>>(             \
         cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid, data_grid, \
         reso, interval, logistic_coef, color, depth, rs, tlist, K, bins)
-    if (ray_pixel != nullptr && tlist == nullptr && bins == nullptr) {
-        // listed rays with uniform samples: one wavefront per ray
-        const unsigned rblocks = (unsigned)((n_rays + kWavesPerBlock - 1) / kWavesPerBlock);
-        if (channels == 32)
-            fused_fwd_ray_kernel<9><<<rblocks, kFusedBlock, 0, s>>>(
-                cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
-                data_grid, reso, interval, logistic_coef, color, depth);
-        else
-            fused_fwd_ray_kernel<16><<<rblocks, kFusedBlock, 0, s>>>(
-                cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
-                data_grid, reso, interval, logistic_coef, color, depth);
-    } else if (channels == 32) {
+    if (channels == 32) {
         if (bins) QSDVR_FWD(9, true);
         else QSDVR_FWD(9, false);
     } else {

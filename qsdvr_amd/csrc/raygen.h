@@ -74,63 +74,6 @@ __device__ __forceinline__ RayCutResult ray_cut_pixel(const qsdvr_camera &c, uin
     return RayCutResult{dir, ori, count};
 }
 
-// ray_cut_pixel computed by a whole wavefront: every lane works the same
-// pixel and tests candidates j = j0 + lane, 64 at a time, with the same
-// float32 operations; the first run of inside candidates (start and length)
-// is read from the ballots.  Same direction, origin and count bit for bit.
-// Called by all 64 lanes with the same arguments.
-__device__ __forceinline__ RayCutResult ray_cut_wave(const qsdvr_camera &c, uint32_t pix,
-                                                     float interval) {
-#pragma clang fp contract(off)
-    float u = (float)(pix % c.width) + 0.5f;
-    float v = (float)(pix / c.width) + 0.5f;
-    f3 n = camera_to_world(c, u, v, c.near_depth);
-    f3 f = camera_to_world(c, u, v, c.far_depth);
-    f3 d{f.x - n.x, f.y - n.y, f.z - n.z};
-    float nrm = sqrtf((d.x * d.x + d.y * d.y) + d.z * d.z);
-    f3 dir{d.x / nrm, d.y / nrm, d.z / nrm};
-    float end;
-    if (dir.x != 0.0f)
-        end = (f.x - n.x) / dir.x;
-    else if (dir.y != 0.0f)
-        end = (f.y - n.y) / dir.y;
-    else
-        end = (f.z - n.z) / dir.z;
-    const uint32_t cand = (uint32_t)(end / interval);
-    const int lane = threadIdx.x % 64;
-    uint32_t count = 0;
-    bool started = false;
-    f3 ori{0.0f, 0.0f, 0.0f};
-    for (uint32_t j0 = 0; j0 < cand; j0 += 64) {
-        const uint32_t j = j0 + (uint32_t)lane;
-        bool ins = false;
-        f3 p{0.0f, 0.0f, 0.0f};
-        if (j < cand) {
-            float fj = (float)j;
-            p = f3{n.x + (dir.x * fj) * interval, n.y + (dir.y * fj) * interval,
-                   n.z + (dir.z * fj) * interval};
-            ins = in_cube(p);
-        }
-        const unsigned long long m = __ballot(ins);
-        int b = 0;  // first candidate of the run in this chunk
-        if (!started) {
-            if (m == 0) continue;
-            b = __ffsll((long long)m) - 1;
-            ori = f3{__shfl(p.x, b, 64), __shfl(p.y, b, 64), __shfl(p.z, b, 64)};
-            started = true;
-        }
-        // candidates b.. of this chunk that are outside (or past the end)
-        const unsigned long long out = ~m & (~0ull << b);
-        if (out == 0) {
-            count += (uint32_t)(64 - b);  // the run continues into the next chunk
-            continue;
-        }
-        count += (uint32_t)(__ffsll((long long)out) - 1 - b);
-        break;
-    }
-    return RayCutResult{dir, ori, count};
-}
-
 struct CellPoint {
     f3 local;
     int32_t cell;  // x + y*R + z*R*R

@@ -101,10 +101,6 @@ def test_fused_matches_op_surface_full_pose():
 
 
 def test_fused_ray_list_matches_full_frame(small):
-    """Listed rays run one wavefront per ray (T as a product scan, colour as
-    a wave sum) where full frames walk one ray per lane: equal to float32
-    rounding, and the listed rays' RayCut (ballots over 64 candidates at a
-    time) gives the same sample counts, so no ray gains or loses a sample."""
     w, cam, grid, data, target, _ = small
     G = grid.cuda()
     D = data.cuda()
@@ -112,10 +108,8 @@ def test_fused_ray_list_matches_full_frame(small):
     g = torch.Generator().manual_seed(11)
     pick = torch.randperm(w.width * w.height, generator=g)[:3000].to(torch.int32)
     c_sub, d_sub = fused.render(G, D, [cam], w.interval, w.s, ray_pixel=pick.cuda())
-    ref_c, ref_d = c_full.cpu()[pick.long()], d_full.cpu()[pick.long()]
-    assert_close(c_sub, ref_c, 1e-6, "listed vs full-frame colour")
-    assert_close(d_sub, ref_d, 1e-6, "listed vs full-frame depth")
-    assert torch.equal(c_sub.cpu() == 0, ref_c == 0)  # the same rays are empty
+    assert torch.equal(c_sub.cpu(), c_full.cpu()[pick.long()])
+    assert torch.equal(d_sub.cpu(), d_full.cpu()[pick.long()])
 
 
 def test_degenerate_cells_keep_gradients_finite(small):
