@@ -70,6 +70,17 @@ def measured_traffic():
         return json.load(f)
 
 
+def measured_utilization():
+    """Issue-rate figures of the fused kernels (VALU busy fraction, MFMA
+    rate) from the rocprofv3 SQ counter passes recorded next to the
+    kernel-trace summaries (None when the record is absent)."""
+    path = os.path.join(ROOT, "profiles", "fused_utilization.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def algorithmic_bytes(R, C, P):
     """Compulsory HBM bytes of one fused backward launch over P rays: read the
     per-ray colour, depth and their gradients (32 B), read both grids once,
@@ -249,7 +260,8 @@ def main():
            "proposal": prop_results,
            "kernel_ms": {k: round(v, 4) for k, v in kern_ms.items()},
            "training_step": train, "large": large,
-           "roofline": roofline, "op_surface_equivalent": op_equiv, "cpu_baseline": cpu}
+           "roofline": roofline, "utilization": measured_utilization(),
+           "op_surface_equivalent": op_equiv, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
