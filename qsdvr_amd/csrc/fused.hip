@@ -303,16 +303,50 @@ __device__ __forceinline__ CellPoint point_at(const RaySetup &ray, float t, floa
     return to_cell(qx, qy, qz, R);
 }
 
-// Render sample j of the ray (cell clamped as clamp_cell does).
-__device__ __forceinline__ CellPoint render_point(const RaySetup &ray, uint32_t j, float interval,
-                                                  float half_reso, int R) {
-    CellPoint rp;
-    if (ray.tl) {
-        const float sj = 0.5f * (ray.tl[j] + ray.tl[j + 1]);
-        rp = point_at(ray, sj, half_reso, R);
-    } else {
-        rp = render_sample(ray.ori, ray.dir, j, interval, half_reso, R);
+// Sample positions of a listed ray.  AHEAD: held ahead of their use, so at
+// step j a = t_j and b = t_{j+1} are in registers and t_{j+2} is in flight,
+// and a step's quadric and row gathers never wait on a load from the list
+// (each lane's list is a row of its own, so a load per step puts a memory
+// round trip in front of every gather of the step).  Otherwise each
+// position is loaded where it is used, which needs no registers: the
+// 53-channel forward then keeps 4 waves per SIMD.  Uniform rays read none.
+template <bool AHEAD>
+struct TWindow {
+    float a = 0.0f, b = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void start(const RaySetup &ray) {
+        if (AHEAD && ray.tl != nullptr) {
+            a = ray.tl[0];
+            b = ray.tl[1];
+            c = ray.renders >= 2 ? ray.tl[2] : 0.0f;
+        }
     }
+    // step j done: t_{j+1}, t_{j+2} move down and t_{j+3} is requested
+    __device__ __forceinline__ void advance(const RaySetup &ray, int j) {
+        if (AHEAD && ray.tl != nullptr) {
+            a = b;
+            b = c;
+            c = (uint32_t)(j + 3) <= ray.renders ? ray.tl[j + 3] : 0.0f;
+        }
+    }
+    // t_j and t_{j+1} at step j
+    __device__ __forceinline__ float t0(const RaySetup &ray, int j) const {
+        return AHEAD ? a : (ray.tl ? ray.tl[j] : 0.0f);
+    }
+    __device__ __forceinline__ float t1(const RaySetup &ray, int j) const {
+        return AHEAD ? b : (ray.tl ? ray.tl[j + 1] : 0.0f);
+    }
+    // position of render sample j (midpoint of t_j and t_{j+1})
+    __device__ __forceinline__ float mid(const RaySetup &ray, int j) const {
+        return 0.5f * (t0(ray, j) + t1(ray, j));
+    }
+};
+
+// Render sample j of the ray (cell clamped as clamp_cell does); sl is its
+// position on a listed ray (TWindow::mid), unused for uniform rays.
+__device__ __forceinline__ CellPoint render_point_t(const RaySetup &ray, uint32_t j, float sl,
+                                                    float interval, float half_reso, int R) {
+    CellPoint rp = ray.tl ? point_at(ray, sl, half_reso, R)
+                          : render_sample(ray.ori, ray.dir, j, interval, half_reso, R);
     clamp_cell(rp, R);
     return rp;
 }
@@ -320,13 +354,13 @@ __device__ __forceinline__ CellPoint render_point(const RaySetup &ray, uint32_t 
 // Render sample j at its cell point rp: normal, appearance (from the staged
 // box when bx.lds is set, else from global memory), shading and distance.
 template <int NC>
-__device__ __forceinline__ void eval_render_at(const RaySetup &ray, uint32_t j, float interval,
-                                               const CellPoint &rp, int R,
+__device__ __forceinline__ void eval_render_at(const RaySetup &ray, uint32_t j, float sl,
+                                               float interval, const CellPoint &rp, int R,
                                                const float *__restrict__ qgrid,
                                                const float *__restrict__ data, const BoxView &bx,
                                                float w[8], float d[3 * NC + 5], SampleEval &e) {
     constexpr int C = 3 * NC + 5;
-    const float sj = ray.tl ? 0.5f * (ray.tl[j] + ray.tl[j + 1]) : ((float)j + 0.5f) * interval;
+    const float sj = ray.tl ? sl : ((float)j + 0.5f) * interval;
     e.rp = rp;
     const Quadric q = load_quadric(qgrid, e.rp.cell);
     e.normal = quadric_normal(q, e.rp.local);
@@ -346,15 +380,16 @@ __device__ __forceinline__ void eval_render_at(const RaySetup &ray, uint32_t j, 
     e.t = fast_sqrt(px * px + py * py + pz * pz);
 }
 
-// Render sample j with per-lane gathers from global memory.
+// Render sample j with per-lane gathers from global memory (sl: position on
+// a listed ray, TWindow::mid).
 template <int NC>
-__device__ __forceinline__ void eval_render_sample(const RaySetup &ray, uint32_t j, float interval,
-                                                   float half_reso, int R,
-                                                   const float *__restrict__ qgrid,
-                                                   const float *__restrict__ data, float w[8],
-                                                   float d[3 * NC + 5], SampleEval &e) {
+__device__ __forceinline__ void eval_render_sample_t(const RaySetup &ray, uint32_t j, float sl,
+                                                     float interval, float half_reso, int R,
+                                                     const float *__restrict__ qgrid,
+                                                     const float *__restrict__ data, float w[8],
+                                                     float d[3 * NC + 5], SampleEval &e) {
     constexpr int C = 3 * NC + 5;
-    const float sj = ray.tl ? 0.5f * (ray.tl[j] + ray.tl[j + 1]) : ((float)j + 0.5f) * interval;
+    const float sj = ray.tl ? sl : ((float)j + 0.5f) * interval;
     e.rp = ray.tl ? point_at(ray, sj, half_reso, R)
                   : render_sample(ray.ori, ray.dir, j, interval, half_reso, R);
     clamp_cell(e.rp, R);
@@ -371,6 +406,17 @@ __device__ __forceinline__ void eval_render_sample(const RaySetup &ray, uint32_t
     e.t = fast_sqrt(px * px + py * py + pz * pz);
 }
 
+// Render sample j, a listed ray's position read from its list.
+template <int NC>
+__device__ __forceinline__ void eval_render_sample(const RaySetup &ray, uint32_t j, float interval,
+                                                   float half_reso, int R,
+                                                   const float *__restrict__ qgrid,
+                                                   const float *__restrict__ data, float w[8],
+                                                   float d[3 * NC + 5], SampleEval &e) {
+    const float sl = ray.tl ? 0.5f * (ray.tl[j] + ray.tl[j + 1]) : 0.0f;
+    eval_render_sample_t<NC>(ray, j, sl, interval, half_reso, R, qgrid, data, w, d, e);
+}
+
 // Cell point of sdf sample j (as sdf_at computes it).
 __device__ __forceinline__ CellPoint ray_sdf_point(const RaySetup &ray, uint32_t j, float interval,
                                                    float half_reso, int R) {
@@ -380,17 +426,28 @@ __device__ __forceinline__ CellPoint ray_sdf_point(const RaySetup &ray, uint32_t
     return cp;
 }
 
-__device__ __forceinline__ float sdf_at(const RaySetup &ray, uint32_t j, float interval,
-                                        float half_reso, int R, float inv_reso,
-                                        const float *__restrict__ qgrid, CellPoint &cp) {
-    cp = ray.tl ? point_at(ray, ray.tl[j], half_reso, R)
+// SDF at sample j; tj is its position on a listed ray (unused otherwise).
+__device__ __forceinline__ float sdf_at_t(const RaySetup &ray, uint32_t j, float tj,
+                                          float interval, float half_reso, int R, float inv_reso,
+                                          const float *__restrict__ qgrid, CellPoint &cp) {
+    cp = ray.tl ? point_at(ray, tj, half_reso, R)
                 : sdf_sample(ray.ori, ray.dir, j, interval, half_reso, R);
     clamp_cell(cp, R);
     return quadric_sdf(load_quadric(qgrid, cp.cell), cp.local, inv_reso);
 }
 
-template <int NC, bool BINS>
-__global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
+__device__ __forceinline__ float sdf_at(const RaySetup &ray, uint32_t j, float interval,
+                                        float half_reso, int R, float inv_reso,
+                                        const float *__restrict__ qgrid, CellPoint &cp) {
+    return sdf_at_t(ray, j, ray.tl ? ray.tl[j] : 0.0f, interval, half_reso, R, inv_reso, qgrid,
+                    cp);
+}
+
+// LISTED: the rays carry explicit sample lists (tlist); the uniform-ray
+// instantiation drops the list code and its registers.  C = 32: at most 128
+// registers per lane, 4 waves per SIMD (as many as the staging LDS allows).
+template <int NC, bool BINS, bool LISTED>
+__global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel(
     const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
     const int32_t *__restrict__ ray_pixel,
     int64_t n_rays, const float *__restrict__ qgrid, const float *__restrict__ data, int R,
@@ -399,7 +456,9 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
     float *__restrict__ bins) {
     constexpr int C = 3 * NC + 5;
     extern __shared__ float4 fwd_lds[];
-    RaySetup ray = setup_ray(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state, tlist, K);
+    RaySetup ray = setup_ray(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state,
+                             LISTED ? tlist : nullptr, K);
+    if constexpr (!LISTED) ray.tl = nullptr;
     const bool live = ray.out >= 0 && ray.renders > 0;
     // Tile waves (rays of one 8x8 pixel tile) stage their appearance rows in
     // LDS; waves of listed rays gather per lane.  The step loop runs
@@ -412,7 +471,11 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
     float dep = 0.0f;
     const int steps = wave_max(live ? (int)ray.renders : 0);
     CellPoint cp;
-    float e0 = live ? expf(-s * sdf_at(ray, 0, interval, half_reso, R, inv_reso, qgrid, cp)) : 1.0f;
+    TWindow<LISTED && C == 32> tw;
+    if (live) tw.start(ray);
+    float e0 = live ? expf(-s * sdf_at_t(ray, 0, tw.t0(ray, 0), interval, half_reso, R, inv_reso,
+                                         qgrid, cp))
+                    : 1.0f;
     float T = 1.0f;
     // 64 equal bins over the ray's sampled segment (training targets).
     // The bin index never decreases along the ray, so each bin is summed
@@ -423,7 +486,8 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
         const bool act = live && j < (int)ray.renders;
         float a = 0.0f, wt = 0.0f, e1 = 0.0f;
         if (act) {
-            e1 = expf(-s * sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp));
+            e1 = expf(-s * sdf_at_t(ray, j + 1, tw.t1(ray, j), interval, half_reso, R, inv_reso,
+                                    qgrid, cp));
             a = fmaxf((e1 - e0) * fast_rcp(1.0f + e1), 0.0f);
             wt = a * T;
         }
@@ -435,7 +499,7 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
             CellPoint rp;
             rp.ix = rp.iy = rp.iz = 0;
             rp.cell = 0;
-            if (need) rp = render_point(ray, (uint32_t)j, interval, half_reso, R);
+            if (need) rp = render_point_t(ray, (uint32_t)j, tw.mid(ray, j), interval, half_reso, R);
             BoxView bx{nullptr, 0, 0, 0};
             if constexpr (C == 32) {
                 if (stage) bx = stage_box<C>(box, data, R, need, rp, lane);
@@ -443,7 +507,8 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
             if (need) {
                 float w[8], d[C];
                 SampleEval ev;
-                eval_render_at<NC>(ray, (uint32_t)j, interval, rp, R, qgrid, data, bx, w, d, ev);
+                eval_render_at<NC>(ray, (uint32_t)j, tw.mid(ray, j), interval, rp, R, qgrid, data,
+                                   bx, w, d, ev);
                 col.x += ev.rgb.x * wt;
                 col.y += ev.rgb.y * wt;
                 col.z += ev.rgb.z * wt;
@@ -462,6 +527,7 @@ __global__ void __launch_bounds__(kFusedBlock) fused_fwd_kernel(
             }
             T = T * (1.0f - a);
             e0 = e1;
+            tw.advance(ray, j);
         }
     }
     if (ray.out < 0) return;
@@ -1117,7 +1183,7 @@ constexpr int tile_channels() {
 // At most 256 registers per lane (2 waves per SIMD, 8 per CU, which the
 // 16.5 KB of LDS per wave also allows): occupancy hides the latency of the
 // per-sample gathers.
-template <int NC>
+template <int NC, bool LISTED>
 __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
     const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
     const int32_t *__restrict__ ray_pixel,
@@ -1136,7 +1202,9 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
     for (int i = lane; i < CP * kGradStride + kCapV * kHalf + 7 * kQBoxCells; i += kWave)
         gt[i] = 0.0f;
 
-    RaySetup ray = setup_ray<1>(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state, tlist, K);
+    RaySetup ray = setup_ray<1>(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state,
+                                LISTED ? tlist : nullptr, K);
+    if constexpr (!LISTED) ray.tl = nullptr;
     const bool live = ray.out >= 0 && ray.renders > 0;
     const int steps = wave_max(live ? (int)ray.renders + 1 : 0);
     if (steps == 0) return;
@@ -1155,7 +1223,12 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
         gC = load3(gcolor, ray.out);
         Ds = depth ? depth[ray.out] : 0.0f;
         gD = gdepth ? gdepth[ray.out] : 0.0f;
-        e0 = expf(-s * sdf_at(ray, 0, interval, half_reso, R, inv_reso, qgrid, cp0));
+    }
+    TWindow<LISTED && C == 32> tw;
+    if (live) {
+        tw.start(ray);
+        e0 = expf(-s * sdf_at_t(ray, 0, tw.t0(ray, 0), interval, half_reso, R, inv_reso, qgrid,
+                                cp0));
     }
     float T = 1.0f;
     f3 pre{0.0f, 0.0f, 0.0f};
@@ -1177,7 +1250,8 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
         float g_sdf0 = carry, g_sdf1 = 0.0f;
         bool app = false;  // render sample j of nonzero weight: normal + appearance gradients
         if (act_r) {
-            float sdf1 = sdf_at(ray, j + 1, interval, half_reso, R, inv_reso, qgrid, cp1);
+            float sdf1 =
+                sdf_at_t(ray, j + 1, tw.t1(ray, j), interval, half_reso, R, inv_reso, qgrid, cp1);
             float e1 = expf(-s * sdf1);
             float inv1 = fast_rcp(1.0f + e1);
             float praw = (e1 - e0) * inv1;
@@ -1190,7 +1264,8 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
             if (praw >= 0.0f && T != 0.0f) {
                 float d[C];
                 SampleEval ev;
-                eval_render_sample<NC>(ray, j, interval, half_reso, R, qgrid, data, w, d, ev);
+                eval_render_sample_t<NC>(ray, j, tw.mid(ray, j), interval, half_reso, R, qgrid,
+                                         data, w, d, ev);
                 pre.x += ev.rgb.x * wt;
                 pre.y += ev.rgb.y * wt;
                 pre.z += ev.rgb.z * wt;
@@ -1223,6 +1298,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
             }
             T = T * (1.0f - a);
             e0 = e1;
+            tw.advance(ray, j);
         }
         const CellPoint sc = cp0;  // sdf sample j
         if (act_s) quadric_sdf_grad(load_quadric(qgrid, sc.cell), sc.local, g_sdf0, inv_reso, gs7);
@@ -1527,17 +1603,23 @@ static int fused_forward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t 
     const float4 *rs = reinterpret_cast<const float4 *>(ray_state);
     int64_t waves = fused_waves(width, height, n_cams, ray_pixel, n_rays);
     unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
-#define QSDVR_FWD(NC, BINS)                                                                  \
-    fused_fwd_kernel<NC, BINS><<<blocks, kFusedBlock, fwd_lds_bytes<NC>(), s>>>(             \
+#define QSDVR_FWD(NC, BINS, LISTED)                                                          \
+    fused_fwd_kernel<NC, BINS, LISTED><<<blocks, kFusedBlock, fwd_lds_bytes<NC>(), s>>>(     \
         cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid, data_grid, \
         reso, interval, logistic_coef, color, depth, rs, tlist, K, bins)
-    if (channels == 32) {
-        if (bins) QSDVR_FWD(9, true);
-        else QSDVR_FWD(9, false);
-    } else {
-        if (bins) QSDVR_FWD(16, true);
-        else QSDVR_FWD(16, false);
-    }
+#define QSDVR_FWD_NC(NC)                          \
+    do {                                          \
+        if (tlist != nullptr) {                   \
+            if (bins) QSDVR_FWD(NC, true, true);  \
+            else QSDVR_FWD(NC, false, true);      \
+        } else {                                  \
+            if (bins) QSDVR_FWD(NC, true, false); \
+            else QSDVR_FWD(NC, false, false);     \
+        }                                         \
+    } while (0)
+    if (channels == 32) QSDVR_FWD_NC(9);
+    else QSDVR_FWD_NC(16);
+#undef QSDVR_FWD_NC
 #undef QSDVR_FWD
     QSDVR_LAUNCHED();
     return QSDVR_OK;
@@ -1582,16 +1664,19 @@ static int fused_backward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t
         return QSDVR_OK;
     }
     unsigned blocks = (unsigned)waves;
-    if (channels == 32)
-        fused_bwd_tile_kernel<9><<<blocks, kWave, tile_lds_bytes<9>(), s>>>(
-            cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
-            data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth,
-            sdf_grid_grad, data_grid_grad, rs, tlist, K);
-    else
-        fused_bwd_tile_kernel<16><<<blocks, kWave, tile_lds_bytes<16>(), s>>>(
-            cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,
-            data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth,
-            sdf_grid_grad, data_grid_grad, rs, tlist, K);
+#define QSDVR_TILE(NC, LISTED)                                                               \
+    fused_bwd_tile_kernel<NC, LISTED><<<blocks, kWave, tile_lds_bytes<NC>(), s>>>(           \
+        cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,        \
+        data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth,      \
+        sdf_grid_grad, data_grid_grad, rs, tlist, K)
+    if (channels == 32) {
+        if (tlist != nullptr) QSDVR_TILE(9, true);
+        else QSDVR_TILE(9, false);
+    } else {
+        if (tlist != nullptr) QSDVR_TILE(16, true);
+        else QSDVR_TILE(16, false);
+    }
+#undef QSDVR_TILE
     QSDVR_LAUNCHED();
     return QSDVR_OK;
 }
