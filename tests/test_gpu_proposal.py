@@ -121,9 +121,13 @@ def _listed_lists(rs, t, R):
             "renderPointList": rp, "renderIndexList": ri.astype(np.int32), "viewDirList": vd}
 
 
-@pytest.mark.parametrize("mode", [fused.SCATTER_TILE, fused.SCATTER_ATOMIC])
-def test_listed_render_matches_oracle(mode):
-    w = scenes.Workload("t", 24, 2, 32, 24, 64, 3, "torus", 0.005)
+@pytest.mark.parametrize("mode", [fused.SCATTER_TILE, fused.SCATTER_ATOMIC, fused.SCATTER_DET])
+@pytest.mark.parametrize("sh_degree", [2, 3])
+def test_listed_render_matches_oracle(mode, sh_degree):
+    """Colour, depth and both grids' gradients of the listed-sample renderer
+    (32 or 53 appearance channels) against the float64 oracle on the same
+    sample lists."""
+    w = scenes.Workload("t", 24, sh_degree, 32, 24, 64, 3, "torus", 0.005)
     grid, data, gen = scenes.build_grids(w, seed=1)
     cams = scenes.orbit_cameras(3, w.width, w.height)[1:2]
     buf, rs = _state(cams, w.interval)
@@ -133,16 +137,18 @@ def test_listed_render_matches_oracle(mode):
     s = 8.0
     G = grid.cuda().requires_grad_(True)
     D = data.cuda().requires_grad_(True)
-    c, _ = proposal.ListedRender.apply(G, D, buf, 1, w.width, w.height, None, rs, t, w.interval,
+    c, d = proposal.ListedRender.apply(G, D, buf, 1, w.width, w.height, None, rs, t, w.interval,
                                        s, mode)
     target = torch.rand((n, 3), generator=gen)
-    ((c - target.cuda()) ** 2).mean().backward()
+    (((c - target.cuda()) ** 2).mean() + 0.01 * d.mean()).backward()
     smp = _listed_lists(rs, t, w.R)
     g64 = grid.double().requires_grad_(True)
     d64 = data.double().requires_grad_(True)
-    ref = orc.render(g64, d64, smp, w.R, s)
-    orc.mse_loss(ref, target.double()).backward()
+    ref, dep = orc.render(g64, d64, smp, w.R, s, with_depth=True,
+                          cam_center=cams[0].translation.astype(np.float64))
+    (orc.mse_loss(ref, target.double()) + 0.01 * dep.mean()).backward()
     assert_close(c, ref, 1e-5, "listed colour")
+    assert_close(d, dep, 1e-5, "listed depth")
     assert_close(G.grad.reshape(-1, 7), g64.grad.reshape(-1, 7), 1e-4, "listed sdf grad")
     assert_close(D.grad, d64.grad, 1e-4, "listed appearance grad")
 
