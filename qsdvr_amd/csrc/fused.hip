@@ -417,13 +417,19 @@ __device__ __forceinline__ void eval_render_sample(const RaySetup &ray, uint32_t
     eval_render_sample_t<NC>(ray, j, sl, interval, half_reso, R, qgrid, data, w, d, e);
 }
 
-// Cell point of sdf sample j (as sdf_at computes it).
-__device__ __forceinline__ CellPoint ray_sdf_point(const RaySetup &ray, uint32_t j, float interval,
-                                                   float half_reso, int R) {
-    CellPoint cp = ray.tl ? point_at(ray, ray.tl[j], half_reso, R)
+// Cell point of sdf sample j (as sdf_at computes it); tj is its position on
+// a listed ray (unused otherwise).
+__device__ __forceinline__ CellPoint ray_sdf_point_t(const RaySetup &ray, uint32_t j, float tj,
+                                                     float interval, float half_reso, int R) {
+    CellPoint cp = ray.tl ? point_at(ray, tj, half_reso, R)
                           : sdf_sample(ray.ori, ray.dir, j, interval, half_reso, R);
     clamp_cell(cp, R);
     return cp;
+}
+
+__device__ __forceinline__ CellPoint ray_sdf_point(const RaySetup &ray, uint32_t j, float interval,
+                                                   float half_reso, int R) {
+    return ray_sdf_point_t(ray, j, ray.tl ? ray.tl[j] : 0.0f, interval, half_reso, R);
 }
 
 // SDF at sample j; tj is its position on a listed ray (unused otherwise).
@@ -1224,11 +1230,22 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
         Ds = depth ? depth[ray.out] : 0.0f;
         gD = gdepth ? gdepth[ray.out] : 0.0f;
     }
+    // QAHEAD: the quadrics of sdf samples j and j + 1 stay in registers, the
+    // latter loaded a step ahead of its use, so neither the step's alpha nor
+    // the sdf sample's gradient waits on a quadric load.  (53 channels have
+    // no registers to spare for them: loaded where used.)
+    constexpr bool QAHEAD = C == 32;
+    Quadric qs0, qs1;
     TWindow<LISTED && C == 32> tw;
     if (live) {
         tw.start(ray);
-        e0 = expf(-s * sdf_at_t(ray, 0, tw.t0(ray, 0), interval, half_reso, R, inv_reso, qgrid,
-                                cp0));
+        cp0 = ray_sdf_point_t(ray, 0, tw.t0(ray, 0), interval, half_reso, R);
+        qs0 = load_quadric(qgrid, cp0.cell);
+        if constexpr (QAHEAD) {
+            cp1 = ray_sdf_point_t(ray, 1, tw.t1(ray, 0), interval, half_reso, R);
+            qs1 = load_quadric(qgrid, cp1.cell);
+        }
+        e0 = expf(-s * quadric_sdf(qs0, cp0.local, inv_reso));
     }
     float T = 1.0f;
     f3 pre{0.0f, 0.0f, 0.0f};
@@ -1250,8 +1267,12 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
         float g_sdf0 = carry, g_sdf1 = 0.0f;
         bool app = false;  // render sample j of nonzero weight: normal + appearance gradients
         if (act_r) {
-            float sdf1 =
-                sdf_at_t(ray, j + 1, tw.t1(ray, j), interval, half_reso, R, inv_reso, qgrid, cp1);
+            float sdf1;  // sdf sample j + 1
+            if constexpr (QAHEAD)
+                sdf1 = quadric_sdf(qs1, cp1.local, inv_reso);
+            else
+                sdf1 = sdf_at_t(ray, j + 1, tw.t1(ray, j), interval, half_reso, R, inv_reso, qgrid,
+                                cp1);
             float e1 = expf(-s * sdf1);
             float inv1 = fast_rcp(1.0f + e1);
             float praw = (e1 - e0) * inv1;
@@ -1301,10 +1322,19 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
             tw.advance(ray, j);
         }
         const CellPoint sc = cp0;  // sdf sample j
-        if (act_s) quadric_sdf_grad(load_quadric(qgrid, sc.cell), sc.local, g_sdf0, inv_reso, gs7);
+        if (act_s)
+            quadric_sdf_grad(QAHEAD ? qs0 : load_quadric(qgrid, sc.cell), sc.local, g_sdf0,
+                             inv_reso, gs7);
         if (act_r) {
             carry = g_sdf1;
             cp0 = cp1;
+            if constexpr (QAHEAD) {
+                qs0 = qs1;
+                if ((uint32_t)(j + 2) <= ray.renders) {  // sdf sample j + 2, for the next step
+                    cp1 = ray_sdf_point_t(ray, j + 2, tw.t1(ray, j + 1), interval, half_reso, R);
+                    qs1 = load_quadric(qgrid, cp1.cell);
+                }
+            }
         }
 
         // quadric gradients of the sdf sample and of the render sample
