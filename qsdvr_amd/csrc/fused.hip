@@ -335,6 +335,9 @@ struct TWindow {
     __device__ __forceinline__ float t1(const RaySetup &ray, int j) const {
         return AHEAD ? b : (ray.tl ? ray.tl[j + 1] : 0.0f);
     }
+    __device__ __forceinline__ float t2(const RaySetup &ray, int j) const {
+        return AHEAD ? c : (ray.tl ? ray.tl[j + 2] : 0.0f);
+    }
     // position of render sample j (midpoint of t_j and t_{j+1})
     __device__ __forceinline__ float mid(const RaySetup &ray, int j) const {
         return 0.5f * (t0(ray, j) + t1(ray, j));
@@ -351,18 +354,18 @@ __device__ __forceinline__ CellPoint render_point_t(const RaySetup &ray, uint32_
     return rp;
 }
 
-// Render sample j at its cell point rp: normal, appearance (from the staged
-// box when bx.lds is set, else from global memory), shading and distance.
+// Render sample j at its cell point rp (quadric q): normal, appearance (from
+// the staged box when bx.lds is set, else from global memory), shading and
+// distance.
 template <int NC>
 __device__ __forceinline__ void eval_render_at(const RaySetup &ray, uint32_t j, float sl,
-                                               float interval, const CellPoint &rp, int R,
-                                               const float *__restrict__ qgrid,
+                                               float interval, const CellPoint &rp,
+                                               const Quadric &q, int R,
                                                const float *__restrict__ data, const BoxView &bx,
                                                float w[8], float d[3 * NC + 5], SampleEval &e) {
     constexpr int C = 3 * NC + 5;
     const float sj = ray.tl ? sl : ((float)j + 0.5f) * interval;
     e.rp = rp;
-    const Quadric q = load_quadric(qgrid, e.rp.cell);
     e.normal = quadric_normal(q, e.rp.local);
     trilinear_weights(e.rp.local, w);
     if constexpr (C == 32) {
@@ -482,6 +485,14 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
     float e0 = live ? expf(-s * sdf_at_t(ray, 0, tw.t0(ray, 0), interval, half_reso, R, inv_reso,
                                          qgrid, cp))
                     : 1.0f;
+    // QAHEAD (32 channels): the quadric of sdf sample j + 1 is loaded a step
+    // ahead of its use, so a step's alpha never waits on a load.
+    constexpr bool QAHEAD = C == 32;
+    Quadric qn;
+    if (QAHEAD && live) {
+        cp = ray_sdf_point_t(ray, 1, tw.t1(ray, 0), interval, half_reso, R);
+        qn = load_quadric(qgrid, cp.cell);
+    }
     float T = 1.0f;
     // 64 equal bins over the ray's sampled segment (training targets).
     // The bin index never decreases along the ray, so each bin is summed
@@ -492,8 +503,16 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
         const bool act = live && j < (int)ray.renders;
         float a = 0.0f, wt = 0.0f, e1 = 0.0f;
         if (act) {
-            e1 = expf(-s * sdf_at_t(ray, j + 1, tw.t1(ray, j), interval, half_reso, R, inv_reso,
-                                    qgrid, cp));
+            if constexpr (QAHEAD) {
+                e1 = expf(-s * quadric_sdf(qn, cp.local, inv_reso));
+                if ((uint32_t)(j + 2) <= ray.renders) {  // sdf sample j + 2, for the next step
+                    cp = ray_sdf_point_t(ray, j + 2, tw.t2(ray, j), interval, half_reso, R);
+                    qn = load_quadric(qgrid, cp.cell);
+                }
+            } else {
+                e1 = expf(-s * sdf_at_t(ray, j + 1, tw.t1(ray, j), interval, half_reso, R,
+                                        inv_reso, qgrid, cp));
+            }
             a = fmaxf((e1 - e0) * fast_rcp(1.0f + e1), 0.0f);
             wt = a * T;
         }
@@ -505,7 +524,11 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
             CellPoint rp;
             rp.ix = rp.iy = rp.iz = 0;
             rp.cell = 0;
-            if (need) rp = render_point_t(ray, (uint32_t)j, tw.mid(ray, j), interval, half_reso, R);
+            Quadric qr;  // the render sample's quadric, requested before the row staging
+            if (need) {
+                rp = render_point_t(ray, (uint32_t)j, tw.mid(ray, j), interval, half_reso, R);
+                qr = load_quadric(qgrid, rp.cell);
+            }
             BoxView bx{nullptr, 0, 0, 0};
             if constexpr (C == 32) {
                 if (stage) bx = stage_box<C>(box, data, R, need, rp, lane);
@@ -513,8 +536,8 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
             if (need) {
                 float w[8], d[C];
                 SampleEval ev;
-                eval_render_at<NC>(ray, (uint32_t)j, tw.mid(ray, j), interval, rp, R, qgrid, data,
-                                   bx, w, d, ev);
+                eval_render_at<NC>(ray, (uint32_t)j, tw.mid(ray, j), interval, rp, qr, R, data, bx,
+                                   w, d, ev);
                 col.x += ev.rgb.x * wt;
                 col.y += ev.rgb.y * wt;
                 col.z += ev.rgb.z * wt;
