@@ -1183,6 +1183,10 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // Four K steps of a 16x16 float32 tile product: lane t supplies
 // A[t % 16][t / 16] and B[t / 16][t % 16] of each 16x16x4 step and holds
 // D[4 (t / 16) + i][t % 16] for i < 4.
+__device__ __forceinline__ floatx4 mfma1(float a, float b, floatx4 d) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, d, 0, 0, 0);
+}
+
 __device__ __forceinline__ floatx4 mfma4(float4 a, float4 b, floatx4 d) {
     d = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, d, 0, 0, 0);
     d = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, d, 0, 0, 0);
@@ -1444,7 +1448,8 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                 __asm__ volatile("v_mov_b32 %0, %1" : "=v"(ln) : "v"(lane));
                 const int g = ln >> 4, m = ln & 15;
                 const int k0 = (2 * g) ^ ((ln >> 1) & 7), k1 = k0 ^ 1;
-                const int nmb = (nv + 15) >> 4;
+                // wave-uniform: the row-block tests below branch on a scalar
+                const int nmb = __builtin_amdgcn_readfirstlane((nv + 15) >> 4);
                 // NBG channel blocks share one staging of the weight columns
                 constexpr int NBG = CP / 16 >= 2 ? 2 : 1;
 #pragma unroll
@@ -1462,18 +1467,39 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                             for (int k = 0; k < 8; ++k) wt[wofs[k]] = w[k];
                         }
                         wave_lds_order();
+                        // gradient fragments of the NBG channel blocks, then
+                        // each weight fragment read once for all of them
+                        float4 b0[NBG], b1[NBG];
 #pragma unroll
                         for (int q = 0; q < NBG; ++q) {
                             const float4 *sb = reinterpret_cast<const float4 *>(
                                 gt + (16 * (nb0 + q) + m) * kGradStride + kHalf * h + 8 * g);
-                            const float4 b0 = sb[0], b1 = sb[1];
+                            b0[q] = sb[0];
+                            b1[q] = sb[1];
+                        }
 #pragma unroll
-                            for (int mb = 0; mb < 4; ++mb) {
-                                if (mb < nmb) {
-                                    const float4 *sa =
-                                        reinterpret_cast<const float4 *>(wt + (16 * mb + m) * kHalf);
-                                    acc[q][mb] = mfma4(sa[k1], b1, mfma4(sa[k0], b0, acc[q][mb]));
-                                }
+                        for (int mb = 0; mb < 4; ++mb) {
+                            if (mb < nmb) {
+                                const float4 *sa =
+                                    reinterpret_cast<const float4 *>(wt + (16 * mb + m) * kHalf);
+                                const float4 a0 = sa[k0], a1 = sa[k1];
+                                // the NBG accumulation chains interleaved
+#pragma unroll
+                                for (int q = 0; q < NBG; ++q) acc[q][mb] = mfma1(a0.x, b0[q].x, acc[q][mb]);
+#pragma unroll
+                                for (int q = 0; q < NBG; ++q) acc[q][mb] = mfma1(a0.y, b0[q].y, acc[q][mb]);
+#pragma unroll
+                                for (int q = 0; q < NBG; ++q) acc[q][mb] = mfma1(a0.z, b0[q].z, acc[q][mb]);
+#pragma unroll
+                                for (int q = 0; q < NBG; ++q) acc[q][mb] = mfma1(a0.w, b0[q].w, acc[q][mb]);
+#pragma unroll
+                                for (int q = 0; q < NBG; ++q) acc[q][mb] = mfma1(a1.x, b1[q].x, acc[q][mb]);
+#pragma unroll
+                                for (int q = 0; q < NBG; ++q) acc[q][mb] = mfma1(a1.y, b1[q].y, acc[q][mb]);
+#pragma unroll
+                                for (int q = 0; q < NBG; ++q) acc[q][mb] = mfma1(a1.z, b1[q].z, acc[q][mb]);
+#pragma unroll
+                                for (int q = 0; q < NBG; ++q) acc[q][mb] = mfma1(a1.w, b1[q].w, acc[q][mb]);
                             }
                         }
                         wave_lds_order();
