@@ -1511,16 +1511,20 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
 #pragma unroll
                     for (int mb = 0; mb < 4; ++mb) {
                         if (mb < nmb) {
+                            // the block's 4 vertex offsets fetched together,
+                            // ahead of its atomics
+                            int vo[4];
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) vo[i] = __shfl(vofs, 16 * mb + 4 * g + i, kWave);
 #pragma unroll
                             for (int i = 0; i < 4; ++i) {
                                 const int v = 16 * mb + 4 * g + i;
-                                const int vo = __shfl(vofs, v, kWave);
 #pragma unroll
                                 for (int q = 0; q < NBG; ++q) {
                                     const int ch = 16 * (nb0 + q) + m;
                                     const float d = acc[q][mb][i];
                                     if (v < nv && ch < C && d != 0.0f)
-                                        atomic_addf(gdata + (vbase + vo) * C + ch, d);
+                                        atomic_addf(gdata + (vbase + vo[i]) * C + ch, d);
                                 }
                             }
                         }
