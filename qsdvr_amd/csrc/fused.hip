@@ -1386,7 +1386,9 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
             hx = wave_max_s(hx); hy = wave_max_s(hy); hz = wave_max_s(hz);
             if (hx < lx) break;  // nothing pending (wave-uniform)
             const int cx = hx - lx + 1, cy = hy - ly + 1, cz = hz - lz + 1;  // cells
-            if ((cx + 1) * (cy + 1) * (cz + 1) > kCapV) {
+            // the tight box holds every pending lane: this round is the last
+            const bool whole = (cx + 1) * (cy + 1) * (cz + 1) <= kCapV;
+            if (!whole) {
                 // Oversize: the slab of the tight box at its low end along the
                 // longest axis whose cross-section leaves room for two vertex
                 // layers, with as many cell layers as fit; else the 3x3x3-cell
@@ -1532,6 +1534,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                 }
             }
             wave_lds_order();
+            if (whole) break;
             pend_r = pend_r && !in_r;
         }
     }
