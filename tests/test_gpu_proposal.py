@@ -121,19 +121,17 @@ def _listed_lists(rs, t, R):
             "renderPointList": rp, "renderIndexList": ri.astype(np.int32), "viewDirList": vd}
 
 
-@pytest.mark.parametrize("mode", [fused.SCATTER_TILE, fused.SCATTER_ATOMIC, fused.SCATTER_DET])
-@pytest.mark.parametrize("sh_degree", [2, 3])
-def test_listed_render_matches_oracle(mode, sh_degree):
-    """Colour, depth and both grids' gradients of the listed-sample renderer
-    (32 or 53 appearance channels) against the float64 oracle on the same
-    sample lists."""
+def _listed_case(mode, sh_degree, K):
     w = scenes.Workload("t", 24, sh_degree, 32, 24, 64, 3, "torus", 0.005)
     grid, data, gen = scenes.build_grids(w, seed=1)
     cams = scenes.orbit_cameras(3, w.width, w.height)[1:2]
     buf, rs = _state(cams, w.interval)
     n = rs.shape[0]
     z = torch.randn((n, 64), generator=torch.Generator().manual_seed(7))
-    t = proposal.sample(z.cuda(), rs, w.interval, K=20)
+    if K >= 3:
+        t = proposal.sample(z.cuda(), rs, w.interval, K=K)
+    else:  # the sampler's lists start at 3 samples; the renderer takes 2 (the endpoints)
+        t = proposal.sample(z.cuda(), rs, w.interval, K=3)[:, [0, 2]].contiguous()
     s = 8.0
     G = grid.cuda().requires_grad_(True)
     D = data.cuda().requires_grad_(True)
@@ -147,10 +145,27 @@ def test_listed_render_matches_oracle(mode, sh_degree):
     ref, dep = orc.render(g64, d64, smp, w.R, s, with_depth=True,
                           cam_center=cams[0].translation.astype(np.float64))
     (orc.mse_loss(ref, target.double()) + 0.01 * dep.mean()).backward()
-    assert_close(c, ref, 1e-5, "listed colour")
-    assert_close(d, dep, 1e-5, "listed depth")
-    assert_close(G.grad.reshape(-1, 7), g64.grad.reshape(-1, 7), 1e-4, "listed sdf grad")
-    assert_close(D.grad, d64.grad, 1e-4, "listed appearance grad")
+    assert_close(c, ref, 1e-5, f"listed colour (K={K})")
+    assert_close(d, dep, 1e-5, f"listed depth (K={K})")
+    assert_close(G.grad.reshape(-1, 7), g64.grad.reshape(-1, 7), 1e-4, f"listed sdf grad (K={K})")
+    assert_close(D.grad, d64.grad, 1e-4, f"listed appearance grad (K={K})")
+
+
+@pytest.mark.parametrize("mode", [fused.SCATTER_TILE, fused.SCATTER_ATOMIC, fused.SCATTER_DET])
+@pytest.mark.parametrize("sh_degree", [2, 3])
+def test_listed_render_matches_oracle(mode, sh_degree):
+    """Colour, depth and both grids' gradients of the listed-sample renderer
+    (32 or 53 appearance channels) against the float64 oracle on the same
+    sample lists."""
+    _listed_case(mode, sh_degree, 20)
+
+
+@pytest.mark.parametrize("mode", [fused.SCATTER_TILE, fused.SCATTER_ATOMIC])
+@pytest.mark.parametrize("K", [2, 3])
+def test_listed_shortest_lists(mode, K):
+    """The shortest lists (one or two render samples per ray), where the
+    sample positions and quadrics held a step ahead run out first."""
+    _listed_case(mode, 2, K)
 
 
 def test_listed_uniform_is_plain_and_bins():
