@@ -1214,8 +1214,8 @@ constexpr int tile_channels() {
 }
 
 // At most 256 registers per lane (2 waves per SIMD, 8 per CU, which the
-// 16.5 KB of LDS per wave also allows): occupancy hides the latency of the
-// per-sample gathers.
+// 18.25 KiB of LDS per wave also allows: 146 of the CU's 160 KiB): occupancy
+// hides the latency of the per-sample gathers.
 template <int NC, bool LISTED>
 __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
     const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
