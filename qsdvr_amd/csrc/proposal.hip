@@ -88,10 +88,29 @@ __global__ void __launch_bounds__(256) pack_kernel(const float *__restrict__ the
     }
 }
 
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef short v2s __attribute__((ext_vector_type(2)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+// two floats -> one dword of two bfloat16, a in the low half (round to
+// nearest even, one v_cvt_pk_bf16_f32)
+__device__ __forceinline__ unsigned pk_bf16(float a, float b) {
+    const v2f f = {a, b};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(f, v2bf));
+}
+
+// relu of two packed bfloat16 (v_pk_max_i16 with 0): read as int16, a
+// bfloat16 is negative exactly when its sign bit is set.  Rounding keeps
+// the sign, so this equals rounding the float relu (up to the sign of 0).
+__device__ __forceinline__ unsigned pk_relu_bf16(unsigned x) {
+    const v2s z = {0, 0};
+    return __builtin_bit_cast(unsigned, __builtin_elementwise_max(__builtin_bit_cast(v2s, x), z));
+}
+
 __device__ __forceinline__ v8s pack8(const float v[8]) {
-    v8bf r;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) r[i] = (__bf16)v[i];
+    const v4u r = {pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3]), pk_bf16(v[4], v[5]),
+                   pk_bf16(v[6], v[7])};
     return __builtin_bit_cast(v8s, r);
 }
 
@@ -145,7 +164,22 @@ __device__ __forceinline__ void pe_chain(float4 sa, float4 sb, float t, int g, f
     }
 }
 
-// relu + bf16 of a layer output into the next layer's B operand
+// relu + bf16 of a layer output into the next layer's B operand (inference:
+// the pair is rounded first and the relu applied to the packed halves)
+__device__ __forceinline__ void to_operand_packed(const v4f D[4], v8s B[2]) {
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        v4u r;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+            r[p] = pk_relu_bf16(pk_bf16(D[2 * kb + (p >> 1)][2 * (p & 1)],
+                                        D[2 * kb + (p >> 1)][2 * (p & 1) + 1]));
+        B[kb] = __builtin_bit_cast(v8s, r);
+    }
+}
+
+// relu + bf16 of a layer output into the next layer's B operand, and the
+// activations at their bf16 values (training)
 __device__ __forceinline__ void to_operand(const v4f D[4], v8s B[2], float act[16]) {
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
@@ -223,25 +257,29 @@ __global__ void __launch_bounds__(kMlpBlock) mlp_fwd_kernel(
             layer2(sA, B, sT, lane, D, zofs);
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
-                to_operand(D[u], B[u], act);
                 if (train) {
+                    to_operand(D[u], B[u], act);
 #pragma unroll
                     for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
                         for (int s = 0; s < 8; ++s)
                             x1s[pt[u] * 64 + hidden_index(kb, g, s)] = act[8 * kb + s];
+                } else {
+                    to_operand_packed(D[u], B[u]);
                 }
             }
             layer2(sA + 4 * 2 * 64, B, sT + 64, lane, D, zofs);
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
-                to_operand(D[u], B[u], act);
                 if (train) {
+                    to_operand(D[u], B[u], act);
 #pragma unroll
                     for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
                         for (int s = 0; s < 8; ++s)
                             x2s[pt[u] * 64 + hidden_index(kb, g, s)] = act[8 * kb + s];
+                } else {
+                    to_operand_packed(D[u], B[u]);
                 }
             }
             layer2(sA + 2 * 4 * 2 * 64, B, sT + 128, lane, D, zofs);
@@ -253,7 +291,12 @@ __global__ void __launch_bounds__(kMlpBlock) mlp_fwd_kernel(
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int f = 16 * mt + 4 * g + q;
-                        const float h = fmaxf(D[u][mt][q], 0.0f);
+                        // inference: relu as med3(x, 0, FLT_MAX), one instruction
+                        // (fmaxf adds a NaN-quieting one in IEEE mode); equal
+                        // to relu for every finite x
+                        const float h = train ? fmaxf(D[u][mt][q], 0.0f)
+                                              : __builtin_amdgcn_fmed3f(D[u][mt][q], 0.0f,
+                                                                        3.40282347e38f);
                         if (train) h3s[pt[u] * 64 + f] = h;
                         z += sT[192 + f] * h;
                     }
