@@ -141,11 +141,13 @@ __device__ __forceinline__ void layer2(const v8s *__restrict__ A /*[4][2][64]*/,
     }
 }
 
-// Positional-encoding chain of axis g at the point ori + dir t (lane group 3
-// carries zeros): [p, sin(pi p), cos(pi p), ..., sin(32 pi p), cos(32 pi p), 0, 0, 0]
-__device__ __forceinline__ void pe_chain(float4 sa, float4 sb, float t, int g, float chain[16]) {
-    const float c = g == 0 ? sa.x + sb.x * t
-                           : (g == 1 ? sa.y + sb.y * t : (g == 2 ? sa.z + sb.z * t : 0.0f));
+// Positional-encoding chain of one axis at the coordinate o + d t (o, d: the
+// axis's components of the ray origin and direction; lane group 3 passes 0,
+// 0): [p, sin(pi p), cos(pi p), ..., sin(32 pi p), cos(32 pi p), 0, 0, 0].
+// Slots 13-15 and the whole chain of lane group 3 meet zero weights in the
+// packed fragments, so only their finiteness matters.
+__device__ __forceinline__ void pe_chain(float o, float d, float t, float chain[16]) {
+    const float c = o + d * t;
     float sn, cs;
     sincospif(c, &sn, &cs);
     chain[0] = c;
@@ -158,10 +160,6 @@ __device__ __forceinline__ void pe_chain(float4 sa, float4 sb, float t, int g, f
         cs = c2;
     }
     chain[13] = chain[14] = chain[15] = 0.0f;
-    if (g == 3) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) chain[i] = 0.0f;
-    }
 }
 
 // relu + bf16 of a layer output into the next layer's B operand (inference:
@@ -226,6 +224,9 @@ __global__ void __launch_bounds__(kMlpBlock) mlp_fwd_kernel(
             continue;
         }
         const float L = count >= 2 ? (float)(count - 1) * interval : 0.0f;
+        // this lane group's axis of the ray (group 3: none)
+        const float po = g == 0 ? sa.x : (g == 1 ? sa.y : (g == 2 ? sa.z : 0.0f));
+        const float pd = g == 0 ? sb.x : (g == 1 ? sb.y : (g == 2 ? sb.z : 0.0f));
 #pragma unroll 1
         for (int tp = 0; tp < kBins / 32; ++tp) {
             // two 16-point tiles: bins 32 tp + j and 32 tp + 16 + j
@@ -236,7 +237,7 @@ __global__ void __launch_bounds__(kMlpBlock) mlp_fwd_kernel(
                 const int bin = 32 * tp + 16 * u + j;
                 const float t = ((float)bin + 0.5f) * (1.0f / kBins) * L;
                 float chain[16];
-                pe_chain(sa, sb, t, g, chain);
+                pe_chain(po, pd, t, chain);
                 B[u][0] = pack8(chain);
                 B[u][1] = pack8(chain + 8);
                 pt[u] = r * kBins + bin;
