@@ -344,6 +344,11 @@ struct AppRecords {
     __device__ __forceinline__ float grad(uint32_t r, int ch) const {
         return aval[(int64_t)r * det_app_stride(C) + 4 + ch];
     }
+    // channels 4q .. 4q + 3 (16-B aligned: the stride and the row offset are
+    // multiples of 4 floats)
+    __device__ __forceinline__ float4 grad4(uint32_t r, int q) const {
+        return *reinterpret_cast<const float4 *>(aval + (int64_t)r * det_app_stride(C) + 4 + 4 * q);
+    }
 };
 
 template <int C>
@@ -416,6 +421,82 @@ __global__ void __launch_bounds__(256) det_app_reduce_kernel(
     }
 }
 
+// det_app_reduce_kernel for C = 32 with 8 lanes per run, each lane four
+// channels (16-B row loads): 8 runs in flight per wavefront instead of 2,
+// for the short runs of random ray batches.  Each (vertex, channel) sum
+// still runs over the run's records in order, so the result is bitwise
+// that of det_app_reduce_kernel<32>.  Needs the rows and gdata 16-B aligned.
+template <class Src>
+__global__ void __launch_bounds__(256) det_app_reduce32x4_kernel(
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ idx, Src src,
+    const uint32_t *__restrict__ bounds, int cls, int cbits, int R, float *__restrict__ gdata) {
+    constexpr int C = 32, GW = 8, NG = kWave / GW;
+    const int64_t lo = bounds[cls], hi = bounds[cls + 1];
+    const int lane = threadIdx.x % kWave;
+    const int grp = lane / GW, q = lane % GW;
+    const uint32_t cmask = (1u << cbits) - 1u;
+    const int64_t RR = (int64_t)R * R;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    for (int64_t p0 = lo + wave * kWave; p0 < hi; p0 += nwaves * kWave) {
+        const int64_t p = p0 + lane;
+        const uint32_t k = p < hi ? key[p] : 0xFFFFFFFFu;
+        const bool boundary = p < hi && (p == lo || key[p - 1] != k);
+        const unsigned long long bmask = __ballot(boundary);
+        if (bmask == 0) continue;  // inside a run that started in an earlier chunk
+        const int last = 63 - __clzll((long long)bmask);
+        const int64_t tail_end = run_end(key, p0 + kWave, hi, __shfl(k, last, kWave));
+        unsigned long long heads = bmask;
+        while (heads != 0) {
+            // group g takes the g-th of the remaining run heads (wave-uniform)
+            int my = -1;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const int h = heads != 0 ? __ffsll((long long)heads) - 1 : -1;
+                if (g == grp) my = h;
+                if (heads != 0) heads &= heads - 1;
+            }
+            const uint32_t hk = __shfl(k, my < 0 ? 0 : my, kWave);
+            if (my >= 0) {
+                const unsigned long long later = bmask & ~((2ull << my) - 1);
+                const int64_t start = p0 + my;
+                const int64_t end = later != 0 ? p0 + (__ffsll((long long)later) - 1) : tail_end;
+                float4 acc[8];
+#pragma unroll
+                for (int v = 0; v < 8; ++v) acc[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll 4
+                for (int64_t i = start; i < end; ++i) {
+                    const uint32_t r = idx[i];
+                    const float4 g = src.grad4(r, q);
+                    float w[8];
+                    trilinear_weights(src.local(r), w);
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) {
+                        acc[v].x += w[v] * g.x;
+                        acc[v].y += w[v] * g.y;
+                        acc[v].z += w[v] * g.z;
+                        acc[v].w += w[v] * g.w;
+                    }
+                }
+                const int64_t cell = hk & cmask;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const int64_t row = cell + (v & 1) + ((v >> 1) & 1) * (int64_t)R + (v >> 2) * RR;
+                    float4 *dst = reinterpret_cast<float4 *>(gdata + row * C + 4 * q);
+                    float4 o = *dst;
+                    o.x += acc[v].x;
+                    o.y += acc[v].y;
+                    o.z += acc[v].z;
+                    o.w += acc[v].w;
+                    *dst = o;
+                }
+            }
+        }
+    }
+}
+
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
 int det_reduce(DetBuffers &b, int64_t T, float *gq, float *gdata, hipStream_t s) {
     if (T == 0) return QSDVR_OK;
     const uint64_t R3 = (uint64_t)b.R * b.R * b.R;
@@ -435,7 +516,11 @@ int det_reduce(DetBuffers &b, int64_t T, float *gq, float *gdata, hipStream_t s)
     const int64_t chunks = (T / 8 + kWave - 1) / kWave;
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 4096));
     for (int cls = 0; cls < 8; ++cls) {
-        if (b.C == 32)
+        if (b.C == 32 && aligned16(b.aval) && aligned16(gdata))
+            det_app_reduce32x4_kernel<<<blocks, 256, 0, s>>>(b.akey_s, b.aidx_s,
+                                                            AppRecords<32>{b.aval}, b.bounds, cls,
+                                                            b.cbits, b.R, gdata);
+        else if (b.C == 32)
             det_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(b.akey_s, b.aidx_s,
                                                              AppRecords<32>{b.aval}, b.bounds, cls,
                                                              b.cbits, b.R, gdata);
@@ -610,6 +695,8 @@ int qsdvr_grid_interpolation_backward_sorted(float *grad_grid, const float *poin
     const unsigned blocks =
         (unsigned)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 4096));
     for (int cls = 0; cls < 8; ++cls) {
+        // (one lane per channel here: a frame's samples sorted by cell make
+        // long, uneven runs, and a wavefront of 8 runs waits for its longest)
         if (channels == 32)
             det_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(
                 b.key_s, val_s, InterpSamples<32>{point, grad_out}, b.bounds, cls, cbits, R,
