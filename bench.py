@@ -40,8 +40,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--op-steps", type=int, default=2, help="steps of the op-surface path (0: skip)")
-    ap.add_argument("--cpu-rays", type=int, default=196608,
+    ap.add_argument("--cpu-rays", type=int, default=65536,
                     help="rays in the CPU oracle sample (0: skip the CPU baseline)")
+    ap.add_argument("--cpu-literal-rays", type=int, default=200,
+                    help="rays of the 64^3 / 100x100 workload timed through the literal per-ray "
+                         "compositing loop (0: skip)")
     ap.add_argument("--scatter", type=int, default=0, help="fused scatter mode (0 tile, 1 atomic)")
     ap.add_argument("--proposal-s", type=str, default="40,5",
                     help="logistic coefficients at which the proposal path is measured "
@@ -99,7 +102,7 @@ def main():
     import numpy as np
     import torch
 
-    from qsdvr_amd import fused, proposal, scenes
+    from qsdvr_amd import _lib, fused, proposal, scenes
     from qsdvr_amd import render as hr
     from qsdvr_amd import qsdvr
     from qsdvr_amd.camera import Camera
@@ -187,6 +190,7 @@ def main():
         torch.cuda.synchronize()
         op_ms = e0.elapsed_time(e1) / args.op_steps
         op_rays = P / (op_ms / 1e3)
+        op_renders = sum(int(cam_objs[1 + i].args.renderCount) for i in range(args.op_steps))
         log(f"op-surface: {op_ms:.3f} ms/step, {op_rays:.4g} rays/s")
         del G2, D2, cam_objs
         torch.cuda.empty_cache()
@@ -222,31 +226,33 @@ def main():
     if dom == "fused_backward" and traffic is not None:
         roofline["traffic"] = traffic["bytes_per_launch"]
         roofline["traffic_source"] = traffic["source"]
+        # measured HBM bytes over this run's launch time: the kernel's real
+        # DRAM rate, against the same peak
+        roofline["traffic_frac"] = round(traffic["bytes_per_launch"] / (kern_ms[dom] / 1e3)
+                                         / 1e9 / HBM_PEAK_GBS, 5)
 
-    # ------------------------------------- op-surface byte equivalent ----
-    # The op-surface form of the same step moves (344 + 20 C) bytes per render
-    # sample plus 150 B per ray plus the grids and gradient grids (4 G / P per
-    # ray) through HBM; at the fused path's rays/s that is the bandwidth the
-    # op-surface form would need.
-    renders = 0
-    for i in range(args.steps):
-        p = (args.warmup + i) % w.n_poses
-        rs = proposal.ray_state(cam_buf[p * CAM_BYTES:(p + 1) * CAM_BYTES], 1, w.width, w.height,
-                                w.interval)
-        cnt = proposal.ray_counts(rs).to(torch.int64)
-        renders += int(torch.clamp(cnt - 1, min=0).sum())
-    s_r = renders / (args.steps * P)
-    G_bytes = 28 * w.R ** 3 + 4 * 32 * (w.R + 1) ** 3
-    b_op = (344 + 20 * 32) * s_r + 150 + 4 * G_bytes / P
-    op_equiv = {"bytes_per_ray": round(b_op, 1), "render_samples_per_ray": round(s_r, 2),
-                "effective_GBs": round(b_op * rays_s / 1e9, 1),
-                "frac_of_peak": round(b_op * rays_s / 1e9 / HBM_PEAK_GBS, 4)}
+    # ------------------------------------------ op-surface HBM fraction ----
+    # The op-surface path (the reference's 14 ops, each reading its inputs
+    # and writing its outputs once) moves B_op = (344 + 20 C) s_r + 150 + 4 G/P
+    # bytes per ray, s_r the render samples per ray of the timed poses (from
+    # RayCut's totals), G the bytes of both grids; its HBM rate is its own
+    # rays/s times B_op.
+    op_surface = None
+    if op_rays:
+        s_r = op_renders / (args.op_steps * P)
+        G_bytes = 28 * w.R ** 3 + 4 * 32 * (w.R + 1) ** 3
+        b_op = (344 + 20 * 32) * s_r + 150 + 4 * G_bytes / P
+        op_surface = {"rays_per_s": round(op_rays, 1), "ms_per_step": round(op_ms, 3),
+                      "render_samples_per_ray": round(s_r, 2),
+                      "algorithmic_bytes_per_ray": round(b_op, 1),
+                      "achieved_GBs": round(b_op * op_rays / 1e9, 1),
+                      "frac": round(b_op * op_rays / 1e9 / HBM_PEAK_GBS, 4)}
 
     # --------------------------------------------------- CPU baseline ----
     cpu = None
     if args.cpu_rays > 0:
         cpu = cpu_baseline(w, cams[0], grid.detach().cpu(), data.detach().cpu(),
-                           targets[0].cpu(), args.cpu_rays)
+                           targets[0].cpu(), args.cpu_rays, args.cpu_literal_rays)
 
     out = {"metric": METRIC, "value": round(rays_s, 1), "unit": "rays/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
@@ -256,12 +262,14 @@ def main():
                       "width": w.width, "height": w.height, "samples": w.n_samples,
                       "poses": w.n_poses, "logistic_coef": w.s, "path": "fused plain",
                       "scatter_mode": args.scatter},
+           "library": _lib.loaded_path(),
            "op_surface_rays_per_s": round(op_rays, 1) if op_rays else None,
+           "op_surface": op_surface,
            "proposal": prop_results,
            "kernel_ms": {k: round(v, 4) for k, v in kern_ms.items()},
            "training_step": train, "large": large,
            "roofline": roofline, "utilization": measured_utilization(),
-           "op_surface_equivalent": op_equiv, "cpu_baseline": cpu}
+           "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
@@ -451,34 +459,94 @@ def measure_training_step(args):
     return out
 
 
-def cpu_baseline(w, cam, grid, data, target, n_rays):
+def host_cores():
+    """(threads to use, description): the CPUs this process may run on
+    (``os.sched_getaffinity``), capped by OMP_NUM_THREADS when it is set,
+    and lscpu's model / socket / core summary."""
+    import subprocess
+
+    avail = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    use = min(avail, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else avail
+    desc = {"affinity_cpus": avail, "omp_num_threads": cap}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket",
+                             "Thread(s) per core", "CPU(s)"):
+                desc[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        desc["lscpu"] = "unavailable"
+    return use, desc
+
+
+def cpu_baseline(w, cam, grid, data, target, n_rays, n_literal):
     """The CPU oracle (render_torch restated; torch CPU, float32,
-    vectorised) timed forward+backward on a fixed subset of one pose's rays."""
+    vectorised) timed on a fixed subset of one pose's rays: ray generation
+    (RayCut + GenerateRayPoints restated) + forward + MSE + backward.  Also
+    the 64^3 / 100x100 / N=128 workload in full, and the literal per-ray
+    compositing loop of render_torch.py:155-167 on a sample of its rays."""
     import numpy as np
     import torch
 
     from oracle import raygen
     from oracle import render as orc
+    from qsdvr_amd import scenes
 
+    threads, cores = host_cores()
+    torch.set_num_threads(threads)
     interval = np.float32(w.interval)
-    k = raygen.ray_cut_kernel(cam.as_dict(), interval)
-    P = k["count"].shape[0]
+    P = w.width * w.height
     sel = np.linspace(0, P - 1, n_rays).astype(np.int64)
-    mask = np.zeros(P, dtype=bool)
-    mask[sel] = True
-    smp = raygen.ray_samples(cam.as_dict(), interval, w.R, mask=mask)
     tgt = target[torch.as_tensor(sel)]
-    threads = torch.get_num_threads()
     G = grid.clone().requires_grad_(True)
     D = data.clone().requires_grad_(True)
     t0 = time.perf_counter()
+    smp = raygen.ray_samples(cam.as_dict(), interval, w.R, pixels=sel)
     col = orc.render(G, D, smp, w.R, w.s)
     orc.mse_loss(col, tgt).backward()
     dt = time.perf_counter() - t0
-    return {"value": round(n_rays / dt, 1), "unit": "rays/s", "cores": threads, "kind": "port",
-            "sample": f"{n_rays} rays (every {P // n_rays}th pixel of pose 0), "
-                      f"{smp['renderCount']} render samples, fwd+bwd, float32",
-            "seconds": round(dt, 2)}
+    out = {"value": round(n_rays / dt, 1), "unit": "rays/s", "cores": threads, "kind": "port",
+           "sample": f"{n_rays} rays (every {P // n_rays}th pixel of pose 0 of "
+                     f"{w.name}), {smp['renderCount']} render samples; ray generation + "
+                     f"fwd + MSE + bwd, float32, torch CPU",
+           "seconds": round(dt, 2), "host": cores}
+    # the 64^3 sphere / SH 1 / 100x100 / N=128 workload, vectorised, in full
+    ws = scenes.WORKLOADS["small"]
+    g1, d1, gen = scenes.build_grids(ws, seed=0)
+    cam1 = scenes.orbit_cameras(1, ws.width, ws.height)[0]
+    t1 = torch.rand((ws.width * ws.height, 3), generator=gen)
+    G1 = g1.clone().requires_grad_(True)
+    D1 = d1.clone().requires_grad_(True)
+    t0 = time.perf_counter()
+    s1 = raygen.ray_samples(cam1.as_dict(), np.float32(ws.interval), ws.R)
+    c1 = orc.render(G1, D1, s1, ws.R, ws.s)
+    orc.mse_loss(c1, t1).backward()
+    dt1 = time.perf_counter() - t0
+    out["small_vectorised"] = {"rays_per_s": round(ws.width * ws.height / dt1, 1),
+                               "seconds": round(dt1, 3), "workload": ws.name}
+    if n_literal > 0:
+        # literal per-ray, per-sample loop (render_torch.py:155-167 with each
+        # ray's colour in its own row), forward + backward, on evenly spaced rays
+        pick = np.linspace(0, ws.width * ws.height - 1, n_literal).astype(np.int64)
+        G1 = g1.clone().requires_grad_(True)
+        D1 = d1.clone().requires_grad_(True)
+        t0 = time.perf_counter()
+        sp = raygen.ray_samples(cam1.as_dict(), np.float32(ws.interval), ws.R, pixels=pick)
+        sdf = orc.sample_sdf(G1, torch.as_tensor(sp["sdfPointList"]),
+                             torch.as_tensor(sp["sdfIndexList"]), ws.R)
+        rp = torch.as_tensor(sp["renderPointList"])
+        ri = torch.as_tensor(sp["renderIndexList"])
+        rgb, _ = orc.shade(orc.sample_normal(G1, rp, ri), torch.as_tensor(sp["viewDirList"]),
+                           orc.grid_interpolation(D1, rp, ri, ws.R))
+        cl = orc.composite_literal(rgb, sdf, sp["rayList"], ws.s)
+        orc.mse_loss(cl, t1[torch.as_tensor(pick)]).backward()
+        dtl = time.perf_counter() - t0
+        out["small_literal_loop"] = {"rays_per_s": round(n_literal / dtl, 2),
+                                     "seconds": round(dtl, 2), "rays": n_literal,
+                                     "workload": ws.name}
+    return out
 
 
 if __name__ == "__main__":

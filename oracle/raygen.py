@@ -51,17 +51,23 @@ def camera_to_world(cam: dict, u: np.ndarray, v: np.ndarray, z) -> tuple:
     return px, py, pz
 
 
-def ray_cut_kernel(cam: dict, interval) -> dict:
-    """kernel/camera.cu:12-58, vectorised over all W*H pixels.
+def ray_cut_kernel(cam: dict, interval, pixels=None) -> dict:
+    """kernel/camera.cu:12-58, vectorised over all W*H pixels (or over the
+    pixel ids ``pixels``, in that order: each pixel's ray depends on its id
+    alone).
 
     Returns dict with float32 ``dir`` [P,3], ``origin`` [P,3] (first inside
     step; zeros when none) and uint32 ``count`` [P] (steps strictly inside
     the +-0.9999 cube).
     """
     W, H = int(cam["width"]), int(cam["height"])
-    P = W * H
     interval = _f(interval)
-    pid = np.arange(P, dtype=np.int64)
+    if pixels is None:
+        pid = np.arange(W * H, dtype=np.int64)
+    else:
+        pid = np.asarray(pixels, dtype=np.int64).reshape(-1)
+        assert pid.size == 0 or (pid.min() >= 0 and pid.max() < W * H)
+    P = pid.shape[0]
     u = (pid % W).astype(F32) + F32(0.5)
     v = (pid // W).astype(F32) + F32(0.5)
     nx, ny, nz = camera_to_world(cam, u, v, cam["near"])
@@ -100,15 +106,17 @@ def ray_cut_kernel(cam: dict, interval) -> dict:
     }
 
 
-def ray_cut(cam: dict, mask: np.ndarray | None, interval):
+def ray_cut(cam: dict, mask: np.ndarray | None, interval, pixels=None):
     """kernel/camera.cu:98-130: kernel + host scan over masked pixels.
 
     Returns (rayList int32 [P,4], origin f32 [P,3], dir f32 [P,3],
     sdfTotal, renderTotal).  Rows of unmasked pixels carry the kernel's raw
     sdfCount in column 0 and zeros elsewhere (the caller zero-fills rayList,
     datasets/utils.py:43-45, and the reference only rewrites masked rows).
+    With ``pixels`` the rows are those pixels' rays, in that order, and the
+    scan runs over them alone.
     """
-    k = ray_cut_kernel(cam, interval)
+    k = ray_cut_kernel(cam, interval, pixels)
     P = k["count"].shape[0]
     if mask is None:
         mask = np.ones(P, dtype=bool)
@@ -175,10 +183,11 @@ def generate_ray_points(rayList, origin, dirs, interval, R):
             out["renderIndex"], out["viewDir"])
 
 
-def ray_samples(cam: dict, interval, R, mask=None) -> dict:
+def ray_samples(cam: dict, interval, R, mask=None, pixels=None) -> dict:
     """RayCut followed by GenerateRayPoints on the masked rows, as the
-    reference's Camera does (datasets/utils.py:16-56)."""
-    ray, ori, dirs, st, rt = ray_cut(cam, mask, interval)
+    reference's Camera does (datasets/utils.py:16-56); with ``pixels``, on
+    those pixels' rays only (in that order)."""
+    ray, ori, dirs, st, rt = ray_cut(cam, mask, interval, pixels)
     P = ray.shape[0]
     m = np.ones(P, dtype=bool) if mask is None else np.asarray(mask, bool).reshape(P)
     ray_m, ori_m, dir_m = ray[m], ori[m], dirs[m]

@@ -91,14 +91,34 @@ def sphere_layers(R: int, radius: float = 1.0, dtype=torch.float64):
     return ones.clone(), ones.clone(), ones.clone(), off
 
 
-def _gather7(grid, idx):
+def compact_index(rows, idx):
+    """Position of each full-grid row number of ``idx`` in ``rows``, the
+    sorted row numbers a compacted grid holds (raises if one is absent).
+
+    A compacted grid is the rows ``rows`` of a full grid gathered in that
+    order; it lets the oracle run on the rows a few rays touch of a grid too
+    large to convert to float64 whole.  The arithmetic is unchanged: only the
+    row lookup goes through this map."""
+    rows = torch.as_tensor(rows, dtype=torch.int64)
+    flat = idx.reshape(-1).long()
+    pos = torch.searchsorted(rows, flat)
+    ok = (pos < rows.numel()) & (rows[pos.clamp(max=max(rows.numel() - 1, 0))] == flat)
+    if not bool(ok.all()):
+        raise KeyError("row missing from the compacted grid")
+    return pos.reshape(idx.shape)
+
+
+def _gather7(grid, idx, rows=None):
     g = grid.reshape(-1, 7)
-    return g[idx.long()]
+    i = idx.long() if rows is None else compact_index(rows, idx)
+    return g[i]
 
 
-def sample_sdf(grid, pts, idx, R):
-    """render_torch.py:77-90 (kernel/surface.cu:54-72): f/||grad f||/R."""
-    q = _gather7(grid, idx)
+def sample_sdf(grid, pts, idx, R, rows=None):
+    """render_torch.py:77-90 (kernel/surface.cu:54-72): f/||grad f||/R.
+    ``rows``: the cell numbers held by a compacted ``grid`` (else the full
+    grid [R,R,R,7])."""
+    q = _gather7(grid, idx, rows)
     x, y, z = pts[:, 0], pts[:, 1], pts[:, 2]
     a0, a1, a2 = q[:, 0] * x, q[:, 1] * y, q[:, 2] * z
     a3, a4, a5, a6 = q[:, 3], q[:, 4], q[:, 5], q[:, 6]
@@ -108,9 +128,9 @@ def sample_sdf(grid, pts, idx, R):
     return f / n / R
 
 
-def sample_normal(grid, pts, idx):
+def sample_normal(grid, pts, idx, rows=None):
     """render_torch.py:91-100: F.normalize(grad f) with eps 1e-12."""
-    q = _gather7(grid, idx)
+    q = _gather7(grid, idx, rows)
     g = torch.stack([2 * q[:, 0] * pts[:, 0] + q[:, 3],
                      2 * q[:, 1] * pts[:, 1] + q[:, 4],
                      2 * q[:, 2] * pts[:, 2] + q[:, 5]], dim=1)
@@ -123,20 +143,33 @@ def _lerp(a, b, w):
     return (1 - w) * a + b * w
 
 
-def grid_interpolation(data, pts, idx, R):
+def vertex_rows(idx, R):
+    """The 8 appearance rows [S, 8] a sample of cell ``idx`` reads:
+    i, i+1, i+R, i+R+1, i+R^2, i+R^2+1, i+R^2+R, i+R^2+R+1."""
+    i = torch.as_tensor(idx).long().reshape(-1, 1)
+    off = torch.tensor([0, 1, R, R + 1, R * R, R * R + 1, R * R + R, R * R + R + 1])
+    return i + off
+
+
+def grid_interpolation(data, pts, idx, R, rows=None):
     """render_torch.py:128-143.  data [(R+1)^3, C]; rows addressed with the
     cell's flat index i = x + yR + zR^2 and strides 1, R, R^2; weights are the
-    raw local coordinates."""
+    raw local coordinates.  ``rows``: the vertex numbers held by a compacted
+    ``data``."""
     d = data.reshape(-1, data.shape[-1])
     i000 = idx.long()
     i010 = i000 + R
     i100 = i000 + R * R
     i110 = i100 + R
-    a00 = _lerp(d[i000], d[i000 + 1], pts[:, 0])
-    a01 = _lerp(d[i010], d[i010 + 1], pts[:, 0])
+    if rows is None:
+        row = lambda i: d[i]                                      # noqa: E731
+    else:
+        row = lambda i: d[compact_index(rows, i)]                 # noqa: E731
+    a00 = _lerp(row(i000), row(i000 + 1), pts[:, 0])
+    a01 = _lerp(row(i010), row(i010 + 1), pts[:, 0])
     a0 = _lerp(a00, a01, pts[:, 1])
-    a10 = _lerp(d[i100], d[i100 + 1], pts[:, 0])
-    a11 = _lerp(d[i110], d[i110 + 1], pts[:, 0])
+    a10 = _lerp(row(i100), row(i100 + 1), pts[:, 0])
+    a11 = _lerp(row(i110), row(i110 + 1), pts[:, 0])
     a1 = _lerp(a10, a11, pts[:, 1])
     return _lerp(a0, a1, pts[:, 2])
 
@@ -254,21 +287,24 @@ def composite_literal(rgb, sdf, rayList, s):
     return torch.stack(rows) if rows else torch.zeros((0, 3), dtype=rgb.dtype)
 
 
-def render(sdf_grid, data, samples: dict, R: int, s: float, with_depth=False, cam_center=None):
+def render(sdf_grid, data, samples: dict, R: int, s: float, with_depth=False, cam_center=None,
+           cell_rows=None, vertex_rows=None):
     """RenderGridTorch.forward (render_torch.py:117-168) on given grids.
 
     ``samples`` holds the ray lists (rayList, sdfPointList, sdfIndexList,
     renderPointList, renderIndexList, viewDirList) as numpy arrays; grids are
-    torch tensors of the caller's dtype (gradients flow to them)."""
+    torch tensors of the caller's dtype (gradients flow to them).  With
+    ``cell_rows`` / ``vertex_rows`` the grids are compacted (``compact_index``).
+    ``cam_center`` is one camera centre [3] or one per render sample [S_r, 3]."""
     dt = sdf_grid.dtype
     sp = torch.as_tensor(samples["sdfPointList"]).to(dt)
     si = torch.as_tensor(samples["sdfIndexList"]).long()
     rp = torch.as_tensor(samples["renderPointList"]).to(dt)
     ri = torch.as_tensor(samples["renderIndexList"]).long()
     vd = torch.as_tensor(samples["viewDirList"]).to(dt)
-    sdf = sample_sdf(sdf_grid, sp, si, R)
-    nrm = sample_normal(sdf_grid, rp, ri)
-    dl = grid_interpolation(data, rp, ri, R)
+    sdf = sample_sdf(sdf_grid, sp, si, R, cell_rows)
+    nrm = sample_normal(sdf_grid, rp, ri, cell_rows)
+    dl = grid_interpolation(data, rp, ri, R, vertex_rows)
     rgb, _ = shade(nrm, vd, dl)
     if not with_depth:
         return composite(rgb, sdf, samples["rayList"], s)

@@ -88,15 +88,16 @@ SIGNATURES = {
 }
 
 _lib = None
+_path = None
 
 
 def load(path: str | os.PathLike | None = None):
-    """Load (once) and return the ctypes library; raises if it is missing."""
-    global _lib
+    """Load (once) and return the ctypes library; raises if it is missing.
+    ``path`` loads another build of the same ABI instead of the in-tree
+    library (``loaded_path()`` reports which one is in use)."""
+    global _lib, _path
     if _lib is not None and path is None:
         return _lib
-    if path is None and os.environ.get("QSDVR_LIB"):
-        path = os.environ["QSDVR_LIB"]  # an alternative build of the same ABI
     p = Path(path) if path is not None else LIB_PATH
     if not p.exists():
         raise RuntimeError(
@@ -110,7 +111,14 @@ def load(path: str | os.PathLike | None = None):
         fn.argtypes = args
         fn.restype = res
     _lib = lib
+    _path = str(p.resolve())
     return lib
+
+
+def loaded_path() -> str:
+    """Absolute path of the library the ops call (loads it if needed)."""
+    load()
+    return _path
 
 
 def call(name: str, *args) -> None:

@@ -676,7 +676,10 @@ int qsdvr_grid_interpolation_backward_sorted(float *grad_grid, const float *poin
                   "qsdvr_grid_interpolation_backward_sorted: null pointer");
     const int R = vertex_reso - 1;
     const int64_t cells = (int64_t)R * R * R;
-    QSDVR_REQUIRE(cells < (1ll << 27), "qsdvr_grid_interpolation_backward_sorted: reso too large");
+    // key = (parity class << cbits) | cell, with class 8 marking an invalid
+    // sample: cbits + 4 bits must fit the 32-bit key
+    QSDVR_REQUIRE(bits_at_least((uint64_t)cells) + 4 <= 32,
+                  "qsdvr_grid_interpolation_backward_sorted: reso too large for 32-bit keys");
     OpBuffers b;
     int st = op_prepare(workspace, workspace_bytes, n, cells,
                         b, "qsdvr_grid_interpolation_backward_sorted");

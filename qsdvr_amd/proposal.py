@@ -31,7 +31,34 @@ NBINS = 64
 
 
 def _n_rays(n_cams, W, H, ray_pixel):
-    return n_cams * W * H if ray_pixel is None else int(ray_pixel.shape[0])
+    if ray_pixel is None:
+        return n_cams * W * H
+    return _t("ray_pixel", ray_pixel, torch.int32, ndim=1).shape[0]
+
+
+def _check_state(rs, n=None):
+    """The ray state is [n, 8] float32 (origin, count bits, direction, 0)."""
+    return _t("ray state", rs, torch.float32, ndim=2, shape=(n, 8)).shape[0]
+
+
+def _check_grids(sdf_grid, data_grid=None):
+    _t("sdf_grid", sdf_grid, torch.float32, ndim=4)
+    R = sdf_grid.shape[0]
+    if tuple(sdf_grid.shape) != (R, R, R, 7):
+        raise ValueError(f"sdf_grid must be [R,R,R,7], got {tuple(sdf_grid.shape)}")
+    if data_grid is not None:
+        _t("data_grid", data_grid, torch.float32, ndim=2, shape=((R + 1) ** 3, None))
+    return R
+
+
+def _check_list(tlist, n):
+    """Fine sample positions [n, K] float32 (K >= 2), one row per ray."""
+    if tlist is None:
+        return 0
+    _t("tlist", tlist, torch.float32, ndim=2, shape=(n, None))
+    if tlist.shape[1] < 2:
+        raise ValueError("tlist needs at least 2 samples per ray")
+    return int(tlist.shape[1])
 
 
 def ray_state(cam_buf, n_cams, W, H, interval, ray_pixel=None):
@@ -51,6 +78,7 @@ def ray_counts(rs):
 def plain_bins(sdf_grid, data_grid, cam_buf, n_cams, W, H, interval, s, ray_pixel=None):
     """(colour [n,3], depth [n], bins [n,64]) of the plain uniform renderer."""
     n = _n_rays(n_cams, W, H, ray_pixel)
+    _check_grids(sdf_grid, data_grid)
     dev = sdf_grid.device
     color = torch.empty((n, 3), dtype=torch.float32, device=dev)
     depth = torch.empty((n,), dtype=torch.float32, device=dev)
@@ -65,7 +93,8 @@ def plain_bins_sdf(sdf_grid, rs, interval, s):
     """bins [n, 64] of the plain uniform renderer's compositing weights for
     the rays of the ray state ``rs``, from the SDF alone (the training
     targets; equal to ``plain_bins``'s to float32 rounding)."""
-    n = rs.shape[0]
+    n = _check_state(rs)
+    _check_grids(sdf_grid)
     bins = torch.empty((n, NBINS), dtype=torch.float32, device=rs.device)
     with _timed("plain_bins"):
         _lib.call("qsdvr_plain_bins", _ptr(rs), n, _ptr(sdf_grid), sdf_grid.shape[0],
@@ -94,7 +123,7 @@ class Proposal:
 
     def logits(self, rs, interval, train_buffers=None):
         """Logits [n_rays, 64] at the bin centres of the rays in ``rs``."""
-        n = rs.shape[0]
+        n = _check_state(rs)
         z = torch.empty((n, NBINS), dtype=torch.float32, device=rs.device)
         acts = train_buffers if train_buffers is not None else (None,) * 4
         with _timed("proposal_mlp"):
@@ -103,9 +132,12 @@ class Proposal:
         return z
 
     def grad(self, rs, interval, target):
-        """(loss, grad [P]) of the per-ray softmax cross-entropy against the
-        bin targets, through the hand-written forward and backward kernels."""
-        n = rs.shape[0]
+        """(loss [n], grad [P], logits) of the proposal loss against the bin
+        targets ``target`` [n, 64]: per ray the log-likelihood of the plain
+        renderer's bin weights under the composited proposal weights
+        w_b = T_b (1 - exp(-softplus(z_b))) plus the pass-through outcome,
+        through the hand-written forward and backward kernels."""
+        n = _check_state(rs)
         dev = rs.device
         acts = [torch.empty((n * NBINS, 64), dtype=torch.float32, device=dev) for _ in range(4)]
         z = self.logits(rs, interval, acts)
@@ -165,7 +197,10 @@ SAMPLE_GAMMA = 1.0
 def sample(z, rs, interval, K=32, eps=SAMPLE_EPS, gamma=SAMPLE_GAMMA):
     """Fine sample positions t [n_rays, K] (increasing, t0 = 0, t_{K-1} = L)
     from the pdf (1 - eps) w^gamma / sum w^gamma + eps / 64."""
-    n = rs.shape[0]
+    n = _check_state(rs)
+    _t("logits", z, torch.float32, ndim=2, shape=(n, NBINS))
+    if K < 3:
+        raise ValueError("the sampler draws at least 3 samples per ray")
     t = torch.empty((n, K), dtype=torch.float32, device=rs.device)
     with _timed("proposal_sample"):
         _lib.call("qsdvr_proposal_sample", _ptr(z), _ptr(rs), n, float(interval), int(K),
@@ -180,8 +215,11 @@ class ListedRender(autograd.Function):
     @staticmethod
     def forward(ctx: Any, sdf_grid, data_grid, cam_buf, n_cams, W, H, ray_pixel, rs, tlist,
                 interval, s, scatter_mode=SCATTER_TILE):
-        n = rs.shape[0]
-        K = int(tlist.shape[1]) if tlist is not None else 0
+        n = _check_state(rs)
+        if _n_rays(n_cams, W, H, ray_pixel) != n:
+            raise ValueError("the ray state must hold one row per ray")
+        K = _check_list(tlist, n)
+        _check_grids(sdf_grid, data_grid)
         dev = sdf_grid.device
         color = torch.empty((n, 3), dtype=torch.float32, device=dev)
         depth = torch.empty((n,), dtype=torch.float32, device=dev)
@@ -206,7 +244,10 @@ class ListedRender(autograd.Function):
         gd = torch.zeros_like(data_grid)
         if gcolor is None:
             gcolor = torch.zeros((n, 3), dtype=torch.float32, device=sdf_grid.device)
+        _t("colour gradient", gcolor.contiguous(), torch.float32, shape=(n, 3))
         gdep = gdepth.contiguous() if gdepth is not None else None
+        if gdep is not None:
+            _t("depth gradient", gdep, torch.float32, ndim=1, shape=(n,))
         if mode == SCATTER_DET:
             with _timed("listed_backward"):
                 backward_det(cam_buf, n_cams, W, H, ray_pixel, rs, tlist, sdf_grid, data_grid,

@@ -159,7 +159,7 @@ def GridInterpolationBackward(dataGrid, pointList, indexList, out, reso):
     R1 = int(reso)
     if dataGrid.numel() != R1 ** 3 * Cc:
         raise ValueError(f"dataGrid has {dataGrid.numel()} elements, expected ({R1})^3 x {Cc}")
-    if SORTED_SCATTER and Cc in (32, 53) and (R1 - 1) % 2 == 0 and n > 0:
+    if interp_backward_path(n, R1, Cc) == "sorted":
         ws_n = _sorted_workspace(n, (R1 - 1) ** 3)
         ws = _workspace(ws_n)
         _lib.call("qsdvr_grid_interpolation_backward_sorted", _ptr(dataGrid), _ptr(pointList),
@@ -260,7 +260,7 @@ def SampleSDFBackward(out, sdfGrid, pointList, indexList, sdfGradGrid, reso):
         raise ValueError("sdfGradGrid must match sdfGrid")
     if float(reso) == 0.0:
         raise ValueError("reso must be nonzero")
-    if SORTED_SCATTER and n > 0:
+    if _sorted_fits(n, cells):
         ws_n = _sorted_workspace(n, cells)
         ws = _workspace(ws_n)
         _lib.call("qsdvr_sample_sdf_backward_sorted", _ptr(out), _ptr(sdfGrid), _ptr(pointList),
@@ -290,7 +290,7 @@ def SampleNormalBackward(out, sdfGrid, pointList, indexList, sdfGradGrid):
     _t("sdfGradGrid", sdfGradGrid, torch.float32)
     if sdfGradGrid.numel() != sdfGrid.numel():
         raise ValueError("sdfGradGrid must match sdfGrid")
-    if SORTED_SCATTER and n > 0:
+    if _sorted_fits(n, cells):
         ws_n = _sorted_workspace(n, cells)
         ws = _workspace(ws_n)
         _lib.call("qsdvr_sample_normal_backward_sorted", _ptr(out), _ptr(sdfGrid),
@@ -324,6 +324,33 @@ def _workspace(nbytes):
 # samples of many rays pile onto the same cells.  False selects the atomic
 # kernels.
 SORTED_SCATTER = True
+
+
+def _bits_at_least(v):
+    b = 1
+    while (1 << b) < v:
+        b += 1
+    return b
+
+
+def _sorted_fits(n, cells):
+    """The sorted-segment kernels key samples by cell in 32 bits and count
+    them in 32 bits; outside those limits the atomic kernels run (they have
+    none)."""
+    return SORTED_SCATTER and 0 < n < 0xFFFFFFFF and 0 < cells < 0x7FFFFFFF
+
+
+def interp_backward_path(n, vertex_reso, channels):
+    """"sorted" or "atomic": which GridInterpolationBackward kernel runs.
+    The sorted form keys a sample by (cell parity class, cell) in 32 bits
+    (3 class bits + an invalid flag above the cell bits) and needs an even
+    cell resolution (cells of one parity class then share no vertex row)
+    and 32 or 53 channels."""
+    R = int(vertex_reso) - 1
+    cells = R ** 3
+    ok = (_sorted_fits(n, cells) and channels in (32, 53) and R >= 2 and R % 2 == 0
+          and _bits_at_least(cells) + 4 <= 32)
+    return "sorted" if ok else "atomic"
 
 
 def _sorted_workspace(n, cells):

@@ -16,6 +16,7 @@ from oracle import mlp as om
 from oracle import render as orc
 from qsdvr_amd import fused, proposal, scenes
 from tests.helpers import assert_close
+from tests.subset_oracle import listed_lists
 
 pytestmark = pytest.mark.gpu
 
@@ -90,35 +91,7 @@ def test_sampler_matches_oracle():
         assert torch.all(t[cnt >= 2][:, 1:] >= t[cnt >= 2][:, :-1])
 
 
-def _listed_lists(rs, t, R):
-    """Sample lists for the oracle renderer from explicit positions, with the
-    kernel's float32 operation order: q = ((o + d t) + 1) R/2."""
-    r = rs.cpu().numpy().astype(np.float32)
-    cnt = proposal.ray_counts(rs).cpu().numpy()
-    t = t.cpu().numpy().astype(np.float32)
-    n, K = t.shape
-    live = cnt >= 2
-    half = np.float32(R) / np.float32(2.0)
-
-    def cells(tt, rows):
-        q = [((r[rows, a][:, None] + r[rows, 4 + a][:, None] * tt) + np.float32(1.0)) * half
-             for a in range(3)]
-        c = [np.floor(x).astype(np.int64) for x in q]
-        loc = [x * np.float32(2.0) - ((2 * ci).astype(np.float32) + np.float32(1.0))
-               for x, ci in zip(q, c)]
-        idx = c[0] + c[1] * R + c[2] * R * R
-        return np.stack(loc, -1).reshape(-1, 3), idx.reshape(-1)
-
-    rows = np.nonzero(live)[0]
-    sp, si = cells(t[rows], rows)
-    mid = (np.float32(0.5) * (t[rows][:, :-1] + t[rows][:, 1:])).astype(np.float32)
-    rp, ri = cells(mid, rows)
-    sc = np.where(live, K, 0)
-    rc = np.where(live, K - 1, 0)
-    ray = np.stack([sc, np.cumsum(sc) - sc, rc, np.cumsum(rc) - rc], 1).astype(np.int32)
-    vd = np.repeat(r[rows, 4:7], K - 1, axis=0)
-    return {"rayList": ray, "sdfPointList": sp, "sdfIndexList": si.astype(np.int32),
-            "renderPointList": rp, "renderIndexList": ri.astype(np.int32), "viewDirList": vd}
+_listed_lists = listed_lists
 
 
 def _listed_case(mode, sh_degree, K):

@@ -107,3 +107,42 @@ def test_composite_t_matches_list_compositing():
     rgb = torch.rand((8, 3), generator=g, dtype=torch.float64)
     ray = np.array([[9, 0, 8, 0]], dtype=np.int32)
     assert torch.allclose(om.composite_t(rgb, sdf, 6.0), orc.composite(rgb, sdf, ray, 6.0)[0])
+
+
+def test_bin_centres_closed_form():
+    """Bin b of a ray with c in-cube steps spans [b L/64, (b+1) L/64) of
+    L = (c - 1) * interval from the first in-cube step, so its centre lies
+    at (b + 0.5) / 64 * L along the unit direction.  With c = 65 (64 render
+    samples) the centres coincide with the render samples of the ray
+    generator, ori + dir * ((b + 0.5) * interval); bin_targets puts render
+    sample b in bin b.  With c = 129 each centre lies midway between render
+    samples 2b and 2b + 1, both of which bin_targets puts in bin b."""
+    from oracle import raygen
+    interval = 0.03
+    g = torch.Generator().manual_seed(9)
+    ori = torch.rand((2, 3), generator=g, dtype=torch.float64) * 0.4 - 0.2
+    d = torch.randn((2, 3), generator=g, dtype=torch.float64)
+    d = d / d.norm(dim=1, keepdim=True)
+    b = torch.arange(64, dtype=torch.float64)
+    p = om.bin_centres(ori, d, torch.tensor([65, 129]), interval)
+    assert torch.allclose(p[0], ori[0] + d[0] * ((b + 0.5) * interval)[:, None], atol=1e-15)
+    assert torch.allclose(p[1], ori[1] + d[1] * ((2 * b + 1) * interval)[:, None], atol=1e-15)
+    # the ray generator's render sample j (cube coordinates) is the c = 65 bin centre j
+    R = 64
+    o32, d32 = ori[:1].numpy().astype(np.float32), d[:1].numpy().astype(np.float32)
+    ray = np.array([[65, 0, 64, 0]], dtype=np.int32)
+    _, _, rp, ri, _ = raygen.generate_ray_points(ray, o32, d32, np.float32(interval), R)
+    ri = ri.astype(np.int64)
+    cell = np.stack([ri % R, (ri // R) % R, ri // (R * R)], 1)
+    pos = (2 * cell + 1 + rp.astype(np.float64)) / R - 1
+    assert np.allclose(pos, p[0].numpy(), atol=2e-6)
+    # bin_targets: one-hot weights at render sample j land in the bin whose centre they straddle
+    eye = torch.eye(64, dtype=torch.float64)
+    assert torch.equal(om.bin_targets(eye, torch.full((64,), 64)), eye)
+    w128 = torch.zeros((128, 128), dtype=torch.float64)
+    w128[torch.arange(128), torch.arange(128)] = 1.0
+    q = om.bin_targets(w128, torch.full((128,), 128))
+    assert torch.equal(q.argmax(1), torch.arange(128) // 2)
+    # degenerate rays (c < 2) get their origin
+    p0 = om.bin_centres(ori, d, torch.tensor([1, 0]), interval)
+    assert torch.equal(p0, ori[:, None, :].expand(2, 64, 3))

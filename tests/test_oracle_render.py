@@ -282,6 +282,76 @@ def test_composite_neus_telescoping():
     assert torch.allclose(c2[0], a * torch.tensor([0.2, 0.5, 0.9], dtype=D))
 
 
+def _axis_camera(cx_world, W=5, near=0.4, far=4.0):
+    """A camera at (cx_world, 0, 0) looking along +x; the centre pixel of an
+    odd-sized image maps exactly onto the optical axis (u + cx = 0)."""
+    from qsdvr_amd import scenes
+    cam = scenes.look_at_camera(np.array([cx_world, 0.0, 0.0]), W, W, near=near, far=far)
+    rot = cam.rotation.reshape(3, 3)
+    assert np.array_equal(rot[:, 2], np.array([1, 0, 0], np.float32))
+    return cam
+
+
+def _plane_grid(R, x0):
+    """Quadric grid whose SDF is the plane x = x0 (positive before it along
+    +x): A = B = C = 0, D = -1, G = R (x0 - cell centre x), so
+    f / |grad f| / R = x0 - x exactly."""
+    g = torch.zeros((R, R, R, 7), dtype=D)
+    xc = (2 * torch.arange(R, dtype=D) + 1) / R - 1
+    g[..., 3] = -1.0
+    g[..., 6] = (R * (x0 - xc)).view(1, 1, R)
+    return g
+
+
+@pytest.mark.parametrize("s", [7.0, 300.0])
+def test_depth_closed_form_planar_sdf(s):
+    """Depth D = sum_i T_i alpha_i t_i with t_i the distance from the camera
+    centre to render sample i, on an axis-aligned ray through a plane.  The
+    SDF falls monotonically along the ray, so alpha never clamps and the
+    weights telescope: T_k = (1 + e_0) / (1 + e_k), w_k = (1 + e_0)
+    (sig_k - sig_{k+1}) with sig_k = 1 / (1 + e_k).  Positions, SDF values
+    and distances are rebuilt here in float64 from the camera geometry, not
+    from the oracle's lists.  At large s the depth is the camera-to-plane
+    distance to within one step.  Measuring t from the in-cube ray origin
+    instead of the camera centre moves D by about 1.6 and fails both."""
+    from oracle import raygen
+    R, N, x0, c0 = 16, 64, 0.13, -2.6
+    cam = _axis_camera(c0)
+    interval = np.float32(2 * math.sqrt(3) / N)
+    pix = np.array([12])                                    # centre pixel of 5x5
+    smp = raygen.ray_samples(cam.as_dict(), interval, R, pixels=pix)
+    grid = _plane_grid(R, x0)
+    data = torch.zeros(((R + 1) ** 3, 32), dtype=D)
+    data[:, 27:30] = 0.5                                    # diffuse
+    data[:, 31] = 1.0                                       # ao, metallic 0
+    centre = cam.translation.astype(np.float64)
+    col, dep = orc.render(grid, data, smp, R, s, with_depth=True, cam_center=centre)
+    # float64 geometry: the ray is x(t) = c0 + t along +x from the camera centre
+    dt = float(interval)
+    cx = float(np.float32(c0))
+    xn = cx + float(np.float32(cam.near))                   # near point, depth = distance
+    j = np.arange(0, 400)
+    xj = xn + j * dt
+    inside = np.abs(xj) < 0.9999
+    j0, cnt = int(np.argmax(inside)), int(inside.sum())
+    assert int(smp["rayList"][0, 0]) == cnt
+    ori = xj[j0]
+    xs = ori + np.arange(cnt) * dt                          # sdf samples
+    tr = (ori - cx) + (np.arange(cnt - 1) + 0.5) * dt       # render-sample distances
+    e = np.exp(-s * (x0 - xs))
+    sig = 1.0 / (1.0 + e)
+    w = (1.0 + e[0]) * (sig[:-1] - sig[1:])
+    assert np.all(w >= 0)
+    d_closed = float((w * tr).sum())
+    assert abs(float(dep[0]) - d_closed) <= 1e-5 * d_closed
+    assert abs(float(col[0, 0]) - 0.5 * w.sum()) <= 1e-6
+    if s >= 100:
+        assert abs(d_closed - (x0 - cx)) <= dt
+        assert abs(float(dep[0]) - (x0 - cx)) <= dt
+    # the mistaken distance (from the in-cube origin) is far off
+    assert abs(float((w * (tr - (ori - cx))).sum()) - float(dep[0])) > 1.0
+
+
 def test_clamp_gradient_passes_at_zero():
     # equal consecutive sdf -> pre-clamp alpha exactly 0; torch.clamp passes
     # the gradient there (the reference CUDA kernel zeroes it)
@@ -320,3 +390,36 @@ def test_finite_differences(fn):
         sdf = (torch.randn(10, generator=g, dtype=D) * 0.3).requires_grad_(True)
         rgb = torch.rand((8, 3), generator=g, dtype=D).requires_grad_(True)
         assert torch.autograd.gradcheck(lambda r, s: orc.composite(r, s, ray, 5.0), (rgb, sdf))
+
+
+def test_compacted_grids_give_the_full_grid_result():
+    """The oracle on the rows a few rays touch (compacted grids, rows looked
+    up through ``compact_index``) equals the oracle on the full grids,
+    bit for bit, and so do the gradients at those rows; the pixel-subset ray
+    generator gives the full frame's rows for those pixels."""
+    from oracle import raygen
+    from qsdvr_amd import scenes
+    w = scenes.Workload("t", 12, 2, 16, 12, 40, 1, "torus", 0.005)
+    cam = scenes.orbit_cameras(3, w.width, w.height)[1]
+    grid, data, gen = scenes.build_grids(w, seed=3)
+    pix = np.arange(5, w.width * w.height, 7)
+    full = raygen.ray_samples(cam.as_dict(), np.float32(w.interval), w.R)
+    sub = raygen.ray_samples(cam.as_dict(), np.float32(w.interval), w.R, pixels=pix)
+    assert np.array_equal(sub["rayList"][:, [0, 2]], full["rayList"][pix][:, [0, 2]])
+    centre = cam.translation.astype(np.float64)
+    g64 = grid.double().requires_grad_(True)
+    d64 = data.double().requires_grad_(True)
+    c_full, d_full = orc.render(g64, d64, full, w.R, w.s, with_depth=True, cam_center=centre)
+    (c_full[pix].sum() + d_full[pix].sum()).backward()
+    cells = np.unique(np.concatenate([sub["sdfIndexList"], sub["renderIndexList"]]))
+    verts = np.unique(orc.vertex_rows(sub["renderIndexList"], w.R).numpy())
+    gc = grid.double().reshape(-1, 7)[cells].clone().requires_grad_(True)
+    dc = data.double()[verts].clone().requires_grad_(True)
+    c_sub, d_sub = orc.render(gc, dc, sub, w.R, w.s, with_depth=True, cam_center=centre,
+                              cell_rows=cells, vertex_rows=verts)
+    assert torch.equal(c_sub, c_full[pix].detach()) and torch.equal(d_sub, d_full[pix].detach())
+    (c_sub.sum() + d_sub.sum()).backward()
+    assert torch.allclose(gc.grad, g64.grad.reshape(-1, 7)[cells], rtol=1e-12, atol=1e-15)
+    assert torch.allclose(dc.grad, d64.grad[verts], rtol=1e-12, atol=1e-15)
+    with pytest.raises(KeyError):
+        orc.compact_index(verts[1:], torch.as_tensor(verts[:1]))

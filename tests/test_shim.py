@@ -49,3 +49,18 @@ def test_camera_args_mirror():
     with pytest.raises(ValueError, match="cuda"):
         qsdvr.RayCut(a, None, torch.zeros((12, 4), dtype=torch.int32), torch.zeros((12, 3)),
                      torch.zeros((12, 3)), 0.1)
+
+
+def test_interpolation_backward_path_choice():
+    """GridInterpolationBackward takes the sorted scatter wherever its
+    32-bit (parity class, cell) keys fit, and the atomic kernel (no size
+    limit) everywhere else -- never an error at sizes the reference runs."""
+    n = 1000
+    assert qsdvr.interp_backward_path(n, 129, 32) == "sorted"
+    assert qsdvr.interp_backward_path(n, 385, 53) == "sorted"
+    assert qsdvr.interp_backward_path(n, 513, 32) == "sorted"    # 2^27 cells: 31-bit keys
+    assert qsdvr.interp_backward_path(n, 1025, 32) == "atomic"   # 2^30 cells: 34-bit keys
+    assert qsdvr.interp_backward_path(n, 128, 32) == "atomic"    # odd cell resolution
+    assert qsdvr.interp_backward_path(n, 129, 40) == "atomic"    # channel count
+    assert qsdvr.interp_backward_path(0, 129, 32) == "atomic"
+    assert qsdvr.interp_backward_path(1 << 32, 129, 32) == "atomic"
