@@ -82,6 +82,22 @@ int qsdvr_ray_cut(const qsdvr_camera *cam, const uint8_t *mask_host,
                   int64_t *sdf_total, int64_t *render_total,
                   qsdvr_stream stream);
 
+/* No counterpart in the reference: qsdvr_ray_cut for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, semantics and results as
+ * qsdvr_ray_cut, but the workspace is a static device array of the library
+ * (8.07 MiB per GPU, held until the process ends) instead of a caller
+ * buffer; the call allocates nothing.  At most
+ * qsdvr_ray_cut_noalloc_max_pixels() = 2^23 pixels (larger: QSDVR_ERANGE,
+ * nothing written).  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams, other host threads) race. */
+int64_t qsdvr_ray_cut_noalloc_max_pixels(void);
+int qsdvr_ray_cut_noalloc(const qsdvr_camera *cam, const uint8_t *mask_host,
+                          int32_t *ray_list, float *origin, float *dir,
+                          float interval, int64_t pixel_count,
+                          int64_t *sdf_total, int64_t *render_total,
+                          qsdvr_stream stream);
+
 /* Replaces GenerateRayPoints (kernel/camera.cu:132-146, bound at
  * kernel/qsdvr.cpp:34, prototype include/camera.h:59).  For each of the
  * ray_count rows of ray_list: sdf samples ori + (dir*j)*interval and render

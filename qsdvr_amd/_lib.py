@@ -44,6 +44,9 @@ SIGNATURES = {
     "qsdvr_ray_cut_workspace_size": ([I64], SZ),
     "qsdvr_ray_cut": ([C.POINTER(CameraStruct), P, P, P, P, F, I64, P, SZ,
                        C.POINTER(I64), C.POINTER(I64), P], C.c_int),
+    "qsdvr_ray_cut_noalloc_max_pixels": ([], I64),
+    "qsdvr_ray_cut_noalloc": ([C.POINTER(CameraStruct), P, P, P, P, F, I64,
+                               C.POINTER(I64), C.POINTER(I64), P], C.c_int),
     "qsdvr_generate_ray_points": ([P, P, P, P, P, P, P, P, I64, I64, I64, F, I32, P], C.c_int),
     "qsdvr_layer_to_grid_workspace_size": ([I32], SZ),
     "qsdvr_layer_to_grid_forward": ([P, P, P, P, P, I32, C.c_int, P, SZ, P], C.c_int),

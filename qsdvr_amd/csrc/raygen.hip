@@ -3,6 +3,7 @@
 // Reference semantics: kernel/camera.cu:12-146.
 #include "raygen.h"
 
+#include <mutex>
 #include <string.h>
 
 namespace qsdvr {
@@ -189,6 +190,46 @@ __global__ void __launch_bounds__(kGenBlock) generate_points_kernel(
     }
 }
 
+// RayCut's workspace: device copy of the host mask + per-tile scan sums +
+// the two totals, each 256-byte aligned.
+constexpr size_t ray_cut_workspace_bytes(int64_t pixel_count) {
+    return ((size_t)pixel_count + 255) / 256 * 256 +
+           (size_t)((pixel_count + kScanTile - 1) / kScanTile + 1) * 16 + 512 + 256;
+}
+
+// ------------------------------------------- allocation-free RayCut ----
+// qsdvr_ray_cut_noalloc keeps RayCut's workspace in this array instead of a
+// caller's buffer: 8.07 MiB, enough for 2^23 pixels (3840x2160 fits), held
+// by every GPU the library runs on until the process ends.  Calls on one
+// GPU share it, so calls that overlap on one GPU race.  Nothing in it needs
+// to start at zero: every word a call reads was written earlier in the
+// same call.
+constexpr int64_t kNoAllocMaxPixels = int64_t(1) << 23;
+constexpr size_t kNoAllocScratchBytes = ray_cut_workspace_bytes(kNoAllocMaxPixels);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the array's host-side name is not a device address).
+int scratch_on_current_device(void **out) {
+    int dev = -1;
+    QSDVR_HIP(hipGetDevice(&dev));
+    QSDVR_REQUIRE(dev >= 0 && dev < kMaxDevices, "qsdvr_ray_cut_noalloc: device %d out of range",
+                  dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        QSDVR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        QSDVR_REQUIRE(p != nullptr, "qsdvr_ray_cut_noalloc: no device address for the workspace");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return QSDVR_OK;
+}
+
 }  // namespace qsdvr
 
 using namespace qsdvr;
@@ -201,9 +242,7 @@ const char *qsdvr_last_error(void) { return g_err; }
 
 size_t qsdvr_ray_cut_workspace_size(int64_t pixel_count) {
     if (pixel_count < 0) return 0;
-    int64_t nb = (pixel_count + kScanTile - 1) / kScanTile;
-    // mask copy + per-tile sums + totals, each 256-byte aligned
-    return ((size_t)pixel_count + 255) / 256 * 256 + (size_t)(nb + 1) * 16 + 512 + 256;
+    return ray_cut_workspace_bytes(pixel_count);
 }
 
 int qsdvr_ray_cut(const qsdvr_camera *cam, const uint8_t *mask_host, int32_t *ray_list,
@@ -260,6 +299,23 @@ int qsdvr_ray_cut(const qsdvr_camera *cam, const uint8_t *mask_host, int32_t *ra
     *sdf_total = (int64_t)host_totals[0];
     *render_total = (int64_t)host_totals[1];
     return QSDVR_OK;
+}
+
+int64_t qsdvr_ray_cut_noalloc_max_pixels(void) { return kNoAllocMaxPixels; }
+
+int qsdvr_ray_cut_noalloc(const qsdvr_camera *cam, const uint8_t *mask_host, int32_t *ray_list,
+                          float *origin, float *dir, float interval, int64_t pixel_count,
+                          int64_t *sdf_total, int64_t *render_total, qsdvr_stream stream) {
+    if (pixel_count < 0 || pixel_count > kNoAllocMaxPixels) {
+        set_error("qsdvr_ray_cut_noalloc: %lld pixels exceed the static workspace (at most %lld)",
+                  (long long)pixel_count, (long long)kNoAllocMaxPixels);
+        return QSDVR_ERANGE;
+    }
+    void *ws = nullptr;
+    int st = scratch_on_current_device(&ws);
+    if (st != QSDVR_OK) return st;
+    return qsdvr_ray_cut(cam, mask_host, ray_list, origin, dir, interval, pixel_count, ws,
+                         kNoAllocScratchBytes, sdf_total, render_total, stream);
 }
 
 int qsdvr_generate_ray_points(float *sdf_point, int32_t *sdf_index, float *render_point,
