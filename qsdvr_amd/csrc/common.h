@@ -79,6 +79,18 @@ __device__ __forceinline__ void store3(float *p, int64_t i, f3 v) {
     q[2] = v.z;
 }
 
+// Workgroups are dealt round-robin over the 8 XCDs (block b runs on XCD
+// b % 8, each XCD with its own L2).  xcd_block() renumbers the launch so
+// that XCD x takes one contiguous range of logical blocks (the first
+// nb % 8 XCDs one block more than the others): neighbouring work items --
+// e.g. the samples of neighbouring rays, which gather the same grid rows --
+// then share an L2.  A bijection on [0, gridDim.x).
+__device__ __forceinline__ int64_t xcd_block() {
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    const int64_t q = nb / 8, r = nb % 8, x = b % 8, k = b / 8;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // Hardware reciprocal, reciprocal square root and square root (v_rcp_f32,
 // v_rsq_f32, v_sqrt_f32: about 1 ulp).  The library is built with IEEE
 // division and square root for the bit-exact ray generation; the per-sample

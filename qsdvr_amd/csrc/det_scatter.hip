@@ -495,6 +495,94 @@ __global__ void __launch_bounds__(256) det_app_reduce32x4_kernel(
     }
 }
 
+// det_app_reduce_kernel for the op surface's InterpSamples with the records
+// of a run fetched eight at a time: the group's first 8 lanes load the next
+// 8 sample indices, every lane then issues the 8 upstream values and points
+// together, so a run costs two memory round trips per 8 records instead of
+// one dependent chain per record.  Each (vertex, channel) sum still runs over
+// the run's records in order, so the result is bitwise that of
+// det_app_reduce_kernel.  The walk loop is uniform across the wavefront
+// (groups whose run is done keep issuing masked loads) so that the index
+// broadcast sees every lane.
+template <int C>
+__global__ void __launch_bounds__(256) op_app_reduce_kernel(
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ idx,
+    const float *__restrict__ pts, const float *__restrict__ up,
+    const uint32_t *__restrict__ bounds, int cls, int cbits, int R, float *__restrict__ gdata) {
+    static_assert(C <= kWave, "one lane per channel");
+    constexpr int NG = C <= kWave / 2 ? 2 : 1;  // runs in flight per wave
+    constexpr int GW = kWave / NG;
+    constexpr int B = 8;                        // records per fetch
+    const int64_t lo = bounds[cls], hi = bounds[cls + 1];
+    const int lane = threadIdx.x % kWave;
+    const int grp = lane / GW, ch = lane % GW;
+    const uint32_t cmask = (1u << cbits) - 1u;
+    const int64_t RR = (int64_t)R * R;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    for (int64_t p0 = lo + wave * kWave; p0 < hi; p0 += nwaves * kWave) {
+        const int64_t p = p0 + lane;
+        const uint32_t k = p < hi ? key[p] : 0xFFFFFFFFu;
+        const bool boundary = p < hi && (p == lo || key[p - 1] != k);
+        const unsigned long long bmask = __ballot(boundary);
+        if (bmask == 0) continue;  // inside a run that started in an earlier chunk
+        const int last = 63 - __clzll((long long)bmask);
+        const int64_t tail_end = run_end(key, p0 + kWave, hi, __shfl(k, last, kWave));
+        unsigned long long heads = bmask;
+        while (heads != 0) {
+            int hs[2] = {-1, -1};
+            hs[0] = __ffsll((long long)heads) - 1;
+            heads &= heads - 1;
+            if (NG == 2 && heads != 0) {
+                hs[1] = __ffsll((long long)heads) - 1;
+                heads &= heads - 1;
+            }
+            const int my = hs[grp];
+            const uint32_t hk = __shfl(k, my < 0 ? 0 : my, kWave);
+            int64_t start = 0, end = 0;
+            if (my >= 0) {
+                const unsigned long long later = bmask & ~((2ull << my) - 1);
+                start = p0 + my;
+                end = later != 0 ? p0 + (__ffsll((long long)later) - 1) : tail_end;
+            }
+            float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            for (int64_t i = start;; i += B) {
+                if (!__any(i < end)) break;
+                // lane ch < B of the group fetches the index of record i + ch
+                const uint32_t rr = (ch < B && i + ch < end) ? idx[i + ch] : 0u;
+                uint32_t r[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) r[b] = __shfl(rr, grp * GW + b, kWave);
+                float g[B];
+                f3 pt[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const bool ok = i + b < end;
+                    g[b] = ok && ch < C ? up[(int64_t)r[b] * C + ch] : 0.0f;
+                    pt[b] = ok ? load3(pts, r[b]) : f3{0.0f, 0.0f, 0.0f};
+                }
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    if (i + b < end) {
+                        float w[8];
+                        trilinear_weights(pt[b], w);
+#pragma unroll
+                        for (int v = 0; v < 8; ++v) acc[v] += w[v] * g[b];
+                    }
+                }
+            }
+            if (my >= 0 && ch < C) {
+                const int64_t cell = hk & cmask;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const int64_t row = cell + (v & 1) + ((v >> 1) & 1) * (int64_t)R + (v >> 2) * RR;
+                    gdata[row * C + ch] += acc[v];
+                }
+            }
+        }
+    }
+}
+
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 int det_reduce(DetBuffers &b, int64_t T, float *gq, float *gdata, hipStream_t s) {
@@ -701,13 +789,11 @@ int qsdvr_grid_interpolation_backward_sorted(float *grad_grid, const float *poin
         // (one lane per channel here: a frame's samples sorted by cell make
         // long, uneven runs, and a wavefront of 8 runs waits for its longest)
         if (channels == 32)
-            det_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(
-                b.key_s, val_s, InterpSamples<32>{point, grad_out}, b.bounds, cls, cbits, R,
-                grad_grid);
+            op_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(b.key_s, val_s, point, grad_out,
+                                                           b.bounds, cls, cbits, R, grad_grid);
         else
-            det_app_reduce_kernel<53><<<blocks, 256, 0, s>>>(
-                b.key_s, val_s, InterpSamples<53>{point, grad_out}, b.bounds, cls, cbits, R,
-                grad_grid);
+            op_app_reduce_kernel<53><<<blocks, 256, 0, s>>>(b.key_s, val_s, point, grad_out,
+                                                           b.bounds, cls, cbits, R, grad_grid);
         QSDVR_LAUNCHED();
     }
     return QSDVR_OK;

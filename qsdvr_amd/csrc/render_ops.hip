@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) interp_fwd32_kernel(const float4 *__restr
                                                            float4 *__restrict__ out, int64_t n,
                                                            int R) {
     const int lane = threadIdx.x % kWave, q = lane & 7, slot = lane >> 3;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t wave = (xcd_block() * blockDim.x + threadIdx.x) / kWave;
     const int64_t cells = (int64_t)R * R * R, RR = (int64_t)R * R;
 #pragma unroll
     for (int it = 0; it < ITER; ++it) {
@@ -106,44 +106,125 @@ __global__ void __launch_bounds__(256) interp_bwd_kernel(float *__restrict__ gda
 }
 
 // ------------------------------------------------------------- Shader ----
+// Lane = sample.  A sample's appearance row (C = 32 or 53 floats) is read
+// and its gradient row written through LDS: the block's rows are one
+// contiguous span, moved with coalesced 16-B accesses and stored in LDS
+// with an odd row pitch, so that each lane then reads or writes its own
+// row without bank conflicts (a lane-per-row access of global memory would
+// touch 64 cache lines per instruction).
+constexpr int kShadeBlock = 256;
+
+template <int C, int P>
+__device__ __forceinline__ void lds_put(float *lds, int e, float v) {
+    const int r = e / C;
+    lds[r * P + (e - r * C)] = v;
+}
+
+template <int C, int P>
+__device__ __forceinline__ float lds_get(const float *lds, int e) {
+    const int r = e / C;
+    return lds[r * P + (e - r * C)];
+}
+
+// rows [row0, row0 + nrows) of the contiguous [*, C] array src -> LDS
+template <int C, int P>
+__device__ __forceinline__ void stage_rows_in(const float *__restrict__ src, int64_t row0,
+                                              int nrows, float *lds) {
+    const float *g = src + row0 * C;
+    const int total = nrows * C;
+    int done = 0;
+    if (((uintptr_t)g & 15u) == 0) {
+        const int n4 = total / 4;
+        const float4 *g4 = reinterpret_cast<const float4 *>(g);
+        for (int e4 = threadIdx.x; e4 < n4; e4 += blockDim.x) {
+            const float4 v = g4[e4];
+            lds_put<C, P>(lds, 4 * e4 + 0, v.x);
+            lds_put<C, P>(lds, 4 * e4 + 1, v.y);
+            lds_put<C, P>(lds, 4 * e4 + 2, v.z);
+            lds_put<C, P>(lds, 4 * e4 + 3, v.w);
+        }
+        done = 4 * n4;
+    }
+    for (int e = done + threadIdx.x; e < total; e += blockDim.x) lds_put<C, P>(lds, e, g[e]);
+}
+
+// LDS -> rows [row0, row0 + nrows) of the contiguous [*, C] array dst
+template <int C, int P>
+__device__ __forceinline__ void stage_rows_out(float *__restrict__ dst, int64_t row0, int nrows,
+                                               const float *lds) {
+    float *g = dst + row0 * C;
+    const int total = nrows * C;
+    int done = 0;
+    if (((uintptr_t)g & 15u) == 0) {
+        const int n4 = total / 4;
+        float4 *g4 = reinterpret_cast<float4 *>(g);
+        for (int e4 = threadIdx.x; e4 < n4; e4 += blockDim.x)
+            g4[e4] = float4{lds_get<C, P>(lds, 4 * e4 + 0), lds_get<C, P>(lds, 4 * e4 + 1),
+                            lds_get<C, P>(lds, 4 * e4 + 2), lds_get<C, P>(lds, 4 * e4 + 3)};
+        done = 4 * n4;
+    }
+    for (int e = done + threadIdx.x; e < total; e += blockDim.x) g[e] = lds_get<C, P>(lds, e);
+}
+
 template <int NC>
-__global__ void __launch_bounds__(256) shader_fwd_kernel(float *__restrict__ out,
-                                                         const float *__restrict__ normal,
-                                                         const float *__restrict__ view,
-                                                         const float *__restrict__ data,
-                                                         float *__restrict__ spec, int64_t n) {
+__global__ void __launch_bounds__(kShadeBlock) shader_fwd_kernel(const float *__restrict__ normal,
+                                                                 const float *__restrict__ view,
+                                                                 const float *__restrict__ data,
+                                                                 float *__restrict__ out,
+                                                                 float *__restrict__ spec,
+                                                                 int64_t n) {
     constexpr int C = 3 * NC + 5;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    constexpr int P = C % 2 == 1 ? C : C + 1;
+    __shared__ float rows[kShadeBlock * P];
+    const int64_t row0 = (int64_t)blockIdx.x * kShadeBlock;
+    const int nrows = (int)min((int64_t)kShadeBlock, n - row0);
+    stage_rows_in<C, P>(data, row0, nrows, rows);
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t >= nrows) return;
+    const int64_t i = row0 + t;
     float d[C];
-    const float *row = data + i * C;
 #pragma unroll
-    for (int k = 0; k < C; ++k) d[k] = row[k];
+    for (int k = 0; k < C; ++k) d[k] = rows[t * P + k];
     ShadeOut o = shade<NC>(load3(normal, i), load3(view, i), d);
     store3(out, i, o.rgb);
     store3(spec, i, o.spec);
 }
 
 template <int NC>
-__global__ void __launch_bounds__(256) shader_bwd_kernel(const float *__restrict__ up,
-                                                         const float *__restrict__ normal,
-                                                         const float *__restrict__ view,
-                                                         const float *__restrict__ data,
-                                                         const float *__restrict__ spec,
-                                                         float *__restrict__ gnormal,
-                                                         float *__restrict__ gdata, int64_t n) {
+__global__ void __launch_bounds__(kShadeBlock) shader_bwd_kernel(const float *__restrict__ up,
+                                                                 const float *__restrict__ normal,
+                                                                 const float *__restrict__ view,
+                                                                 const float *__restrict__ data,
+                                                                 const float *__restrict__ spec,
+                                                                 float *__restrict__ gnormal,
+                                                                 float *__restrict__ gdata,
+                                                                 int64_t n) {
     constexpr int C = 3 * NC + 5;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    constexpr int P = C % 2 == 1 ? C : C + 1;
+    __shared__ float rows[kShadeBlock * P];
+    const int64_t row0 = (int64_t)blockIdx.x * kShadeBlock;
+    const int nrows = (int)min((int64_t)kShadeBlock, n - row0);
+    stage_rows_in<C, P>(data, row0, nrows, rows);
+    __syncthreads();
+    const int t = threadIdx.x;
+    const bool live = t < nrows;
+    const int64_t i = row0 + t;
     float d[C], dd[C];
-    const float *row = data + i * C;
+    if (live) {
 #pragma unroll
-    for (int k = 0; k < C; ++k) d[k] = row[k];
-    f3 gn = shade_vjp<NC>(load3(normal, i), load3(view, i), d, load3(spec, i), load3(up, i), dd);
-    store3(gnormal, i, gn);
-    float *orow = gdata + i * C;
+        for (int k = 0; k < C; ++k) d[k] = rows[t * P + k];
+        f3 gn = shade_vjp<NC>(load3(normal, i), load3(view, i), d, load3(spec, i), load3(up, i),
+                              dd);
+        store3(gnormal, i, gn);
+    }
+    __syncthreads();
+    if (live) {
 #pragma unroll
-    for (int k = 0; k < C; ++k) orow[k] = dd[k];
+        for (int k = 0; k < C; ++k) rows[t * P + k] = dd[k];
+    }
+    __syncthreads();
+    stage_rows_out<C, P>(gdata, row0, nrows, rows);
 }
 
 // -------------------------------------------------------- RayAggregate ----
@@ -345,11 +426,11 @@ int qsdvr_shader_forward(float *out, const float *normal, const float *view_dir,
                   "qsdvr_shader_forward: null pointer");
     hipStream_t s = as_stream(stream);
     if (channels == 32)
-        shader_fwd_kernel<9><<<grid_1d(n, 256), 256, 0, s>>>(out, normal, view_dir, data, specular,
-                                                             n);
+        shader_fwd_kernel<9><<<grid_1d(n, kShadeBlock), kShadeBlock, 0, s>>>(
+            normal, view_dir, data, out, specular, n);
     else
-        shader_fwd_kernel<16><<<grid_1d(n, 256), 256, 0, s>>>(out, normal, view_dir, data,
-                                                              specular, n);
+        shader_fwd_kernel<16><<<grid_1d(n, kShadeBlock), kShadeBlock, 0, s>>>(
+            normal, view_dir, data, out, specular, n);
     QSDVR_LAUNCHED();
     return QSDVR_OK;
 }
@@ -365,11 +446,11 @@ int qsdvr_shader_backward(const float *grad_out, const float *normal, const floa
                   "qsdvr_shader_backward: null pointer");
     hipStream_t s = as_stream(stream);
     if (channels == 32)
-        shader_bwd_kernel<9><<<grid_1d(n, 256), 256, 0, s>>>(grad_out, normal, view_dir, data,
-                                                             specular, normal_grad, data_grad, n);
+        shader_bwd_kernel<9><<<grid_1d(n, kShadeBlock), kShadeBlock, 0, s>>>(
+            grad_out, normal, view_dir, data, specular, normal_grad, data_grad, n);
     else
-        shader_bwd_kernel<16><<<grid_1d(n, 256), 256, 0, s>>>(grad_out, normal, view_dir, data,
-                                                              specular, normal_grad, data_grad, n);
+        shader_bwd_kernel<16><<<grid_1d(n, kShadeBlock), kShadeBlock, 0, s>>>(
+            grad_out, normal, view_dir, data, specular, normal_grad, data_grad, n);
     QSDVR_LAUNCHED();
     return QSDVR_OK;
 }

@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(256) sample_sdf_fwd_kernel(float *__restrict__
                                                              const int32_t *__restrict__ idx,
                                                              int64_t n, int64_t cells,
                                                              float inv_reso) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int64_t c = idx[i];
     if (c < 0 || c >= cells) {
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(256) sample_normal_fwd_kernel(float *__restric
                                                                 const float *__restrict__ pts,
                                                                 const int32_t *__restrict__ idx,
                                                                 int64_t n, int64_t cells) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int64_t c = idx[i];
     f3 r;

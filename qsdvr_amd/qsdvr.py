@@ -260,7 +260,7 @@ def SampleSDFBackward(out, sdfGrid, pointList, indexList, sdfGradGrid, reso):
         raise ValueError("sdfGradGrid must match sdfGrid")
     if float(reso) == 0.0:
         raise ValueError("reso must be nonzero")
-    if _sorted_fits(n, cells):
+    if quadric_backward_path(n, sdfGrid) == "sorted":
         ws_n = _sorted_workspace(n, cells)
         ws = _workspace(ws_n)
         _lib.call("qsdvr_sample_sdf_backward_sorted", _ptr(out), _ptr(sdfGrid), _ptr(pointList),
@@ -290,7 +290,7 @@ def SampleNormalBackward(out, sdfGrid, pointList, indexList, sdfGradGrid):
     _t("sdfGradGrid", sdfGradGrid, torch.float32)
     if sdfGradGrid.numel() != sdfGrid.numel():
         raise ValueError("sdfGradGrid must match sdfGrid")
-    if _sorted_fits(n, cells):
+    if quadric_backward_path(n, sdfGrid) == "sorted":
         ws_n = _sorted_workspace(n, cells)
         ws = _workspace(ws_n)
         _lib.call("qsdvr_sample_normal_backward_sorted", _ptr(out), _ptr(sdfGrid),
@@ -317,13 +317,16 @@ def _workspace(nbytes):
     return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device="cuda")
 
 
-# The three accumulating backward ops (GridInterpolationBackward,
-# SampleSDFBackward, SampleNormalBackward) run as sorted-segment sums: the
-# same result as one float atomic per sample and coefficient, bitwise
-# reproducible and several times faster on per-pose sample lists, where the
-# samples of many rays pile onto the same cells.  False selects the atomic
-# kernels.
-SORTED_SCATTER = True
+# How the three accumulating backward ops (GridInterpolationBackward,
+# SampleSDFBackward, SampleNormalBackward) add into the gradient grid; both
+# give the sums of the reference's one float atomic per sample, coefficient
+# and channel (kernel/render.cu:121-152, kernel/surface.cu:74-146):
+#   "sorted": sorted-segment sums in a fixed order -- bitwise reproducible
+#             and several times faster on per-pose sample lists, where the
+#             samples of many rays pile onto the same cells;
+#   "atomic": one atomic per sample, coefficient and channel.
+# Sizes the sorted form does not support fall back to "atomic".
+SCATTER = "sorted"
 
 
 def _bits_at_least(v):
@@ -335,21 +338,27 @@ def _bits_at_least(v):
 
 def _sorted_fits(n, cells):
     """The sorted-segment kernels key samples by cell in 32 bits and count
-    them in 32 bits; outside those limits the atomic kernels run (they have
-    none)."""
-    return SORTED_SCATTER and 0 < n < 0xFFFFFFFF and 0 < cells < 0x7FFFFFFF
+    them in 32 bits."""
+    return 0 < n < 0xFFFFFFFF and 0 < cells < 0x7FFFFFFF
 
 
 def interp_backward_path(n, vertex_reso, channels):
-    """"sorted" or "atomic": which GridInterpolationBackward kernel runs.
-    The sorted form keys a sample by (cell parity class, cell) in 32 bits
-    (3 class bits + an invalid flag above the cell bits) and needs an even
-    cell resolution (cells of one parity class then share no vertex row)
-    and 32 or 53 channels."""
+    """"sorted" or "atomic": which GridInterpolationBackward kernel runs for
+    ``SCATTER``.  The sorted form keys a sample by (cell parity class, cell)
+    in 32 bits (3 class bits + an invalid flag above the cell bits) and
+    needs an even cell resolution (cells of one parity class then share no
+    vertex row) and 32 or 53 channels."""
     R = int(vertex_reso) - 1
     cells = R ** 3
-    ok = (_sorted_fits(n, cells) and channels in (32, 53) and R >= 2 and R % 2 == 0
-          and _bits_at_least(cells) + 4 <= 32)
+    ok = (SCATTER == "sorted" and _sorted_fits(n, cells) and channels in (32, 53) and R >= 2
+          and R % 2 == 0 and _bits_at_least(cells) + 4 <= 32)
+    return "sorted" if ok else "atomic"
+
+
+def quadric_backward_path(n, sdfGrid):
+    """"sorted" or "atomic": which SampleSDF / SampleNormal backward kernel
+    runs for ``SCATTER``."""
+    ok = SCATTER == "sorted" and _sorted_fits(n, sdfGrid.numel() // 7)
     return "sorted" if ok else "atomic"
 
 

@@ -211,10 +211,10 @@ def test_sorted_scatters_match_atomics_on_a_full_pose(monkeypatch):
     cc = Camera(a, w.interval, w.R)
     cc.GenerateRayPoints(None)
     inp = cc.GetRayPoints()
-    monkeypatch.setattr(qsdvr, "SORTED_SCATTER", True)
+    monkeypatch.setattr(qsdvr, "SCATTER", "sorted")
     s1 = _three_scatters(inp, grid, data, w.R, 5)
     s2 = _three_scatters(inp, grid, data, w.R, 5)
-    monkeypatch.setattr(qsdvr, "SORTED_SCATTER", False)
+    monkeypatch.setattr(qsdvr, "SCATTER", "atomic")
     at = _three_scatters(inp, grid, data, w.R, 5)
     for name, x, y, z in zip(("SampleSDF", "SampleNormal", "GridInterpolation"), s1, s2, at):
         assert torch.equal(x, y), f"{name}: sorted scatter differs between two runs"
@@ -223,8 +223,11 @@ def test_sorted_scatters_match_atomics_on_a_full_pose(monkeypatch):
         assert float(x.abs().sum()) > 0
 
 
-def test_sorted_scatters_skip_invalid_indices_and_odd_resolution(monkeypatch):
-    R = 9  # odd: GridInterpolation falls back to the atomic kernel
+@pytest.mark.parametrize("R", [9, 10])
+def test_sorted_scatters_skip_invalid_indices_at_any_resolution(monkeypatch, R):
+    """Random samples with out-of-range cell indices (skipped), at an odd
+    resolution (the sorted GridInterpolation falls back to the atomic
+    kernel) and an even one."""
     g = torch.Generator(device="cuda").manual_seed(8)
     n = 5000
     idx = torch.randint(-3, R ** 3 + 3, (n,), generator=g, device="cuda").to(torch.int32)
@@ -234,11 +237,37 @@ def test_sorted_scatters_skip_invalid_indices_and_odd_resolution(monkeypatch):
     data = torch.zeros(((R + 1) ** 3, 32), device="cuda")
     smp = {"sdfPointList": pts, "sdfIndexList": idx, "renderPointList": pts,
            "renderIndexList": idx}
-    monkeypatch.setattr(qsdvr, "SORTED_SCATTER", True)
+    monkeypatch.setattr(qsdvr, "SCATTER", "sorted")
     s = _three_scatters(smp, grid, data, R, 9)
-    monkeypatch.setattr(qsdvr, "SORTED_SCATTER", False)
+    monkeypatch.setattr(qsdvr, "SCATTER", "atomic")
     a = _three_scatters(smp, grid, data, R, 9)
     for x, y in zip(s, a):
         assert_close(x.reshape(-1, 7) if x.numel() % 7 == 0 and x.dim() == 4 else x,
                      y.reshape(-1, 7) if y.numel() % 7 == 0 and y.dim() == 4 else y, BWD,
                      "sorted vs atomic with invalid indices")
+
+
+@pytest.mark.parametrize("C", [32, 53])
+def test_sorted_interpolation_backward_matches_oracle(monkeypatch, C):
+    """The sorted GridInterpolationBackward against the oracle's adjoint
+    (float64 autograd of render_torch.py:128-143) on random samples that
+    pile onto a few bricks, including the x = R rows that alias the next
+    y-row's x = 0."""
+    from oracle import render as orc
+    R = 12
+    g = torch.Generator().manual_seed(21)
+    n = 20000
+    cx = torch.randint(R - 5, R, (n,), generator=g)
+    cy = torch.randint(0, R, (n,), generator=g)
+    cz = torch.randint(3, 9, (n,), generator=g)
+    idx = (cx + cy * R + cz * R * R).to(torch.int32)
+    pts = torch.rand((n, 3), generator=g) * 2 - 1
+    up = torch.randn((n, C), generator=g)
+    d64 = torch.zeros(((R + 1) ** 3, C), dtype=torch.float64, requires_grad=True)
+    out = orc.grid_interpolation(d64, pts.double(), idx, R)
+    (out * up.double()).sum().backward()
+    monkeypatch.setattr(qsdvr, "SCATTER", "sorted")
+    gd = torch.zeros(((R + 1) ** 3, C), device="cuda")
+    qsdvr.GridInterpolationBackward(gd, pts.cuda(), idx.cuda(), up.cuda(), R + 1)
+    torch.cuda.synchronize()
+    assert_close(gd, d64.grad, BWD, f"sorted GridInterpolationBackward (C={C})")

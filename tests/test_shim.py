@@ -51,11 +51,13 @@ def test_camera_args_mirror():
                      torch.zeros((12, 3)), 0.1)
 
 
-def test_interpolation_backward_path_choice():
-    """GridInterpolationBackward takes the sorted scatter wherever its
-    32-bit (parity class, cell) keys fit, and the atomic kernel (no size
-    limit) everywhere else -- never an error at sizes the reference runs."""
+def test_interpolation_backward_path_choice(monkeypatch):
+    """GridInterpolationBackward takes the sorted scatter wherever its 32-bit
+    (parity class, cell) keys fit at an even resolution, and the atomic
+    kernel (no size limit) everywhere else -- never an error at sizes the
+    reference runs."""
     n = 1000
+    assert qsdvr.SCATTER == "sorted"
     assert qsdvr.interp_backward_path(n, 129, 32) == "sorted"
     assert qsdvr.interp_backward_path(n, 385, 53) == "sorted"
     assert qsdvr.interp_backward_path(n, 513, 32) == "sorted"    # 2^27 cells: 31-bit keys
@@ -64,3 +66,14 @@ def test_interpolation_backward_path_choice():
     assert qsdvr.interp_backward_path(n, 129, 40) == "atomic"    # channel count
     assert qsdvr.interp_backward_path(0, 129, 32) == "atomic"
     assert qsdvr.interp_backward_path(1 << 32, 129, 32) == "atomic"
+    monkeypatch.setattr(qsdvr, "SCATTER", "atomic")
+    assert qsdvr.interp_backward_path(n, 129, 32) == "atomic"
+
+
+def test_quadric_backward_path_choice(monkeypatch):
+    g = torch.zeros((16, 16, 16, 7))
+    assert qsdvr.quadric_backward_path(100, g) == "sorted"
+    assert qsdvr.quadric_backward_path(0, g) == "atomic"
+    assert qsdvr.quadric_backward_path(1 << 32, g) == "atomic"
+    monkeypatch.setattr(qsdvr, "SCATTER", "atomic")
+    assert qsdvr.quadric_backward_path(100, g) == "atomic"
