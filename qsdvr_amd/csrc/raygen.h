@@ -19,7 +19,7 @@ struct RayCutResult {
 
 // kernel/camera.cu:3-10: ((u + cx) / fx) * z, rotation rows summed left to
 // right, then translation.
-__device__ __forceinline__ f3 camera_to_world(const qsdvr_camera &c, float u, float v, float z) {
+__host__ __device__ __forceinline__ f3 camera_to_world(const qsdvr_camera &c, float u, float v, float z) {
 #pragma clang fp contract(off)
     float x = ((u + c.cx) / c.fx) * z;
     float y = ((v + c.cy) / c.fy) * z;
@@ -30,18 +30,84 @@ __device__ __forceinline__ f3 camera_to_world(const qsdvr_camera &c, float u, fl
     return p;
 }
 
-__device__ __forceinline__ bool in_cube(f3 p) {
+__host__ __device__ __forceinline__ bool in_cube(f3 p) {
     const float b = 0.9999f;  // kernel/camera.cu:46
     return p.x < b && p.x > -b && p.y < b && p.y > -b && p.z < b && p.z > -b;
 }
 
+// Candidate step j of a ray: near + (dir*j)*interval (kernel/camera.cu:45).
+__host__ __device__ __forceinline__ f3 cut_step(f3 n, f3 dir, float interval, uint32_t j) {
+#pragma clang fp contract(off)
+    float fj = (float)j;
+    return f3{n.x + (dir.x * fj) * interval, n.y + (dir.y * fj) * interval,
+              n.z + (dir.z * fj) * interval};
+}
+
+// The reference's walk over the candidate steps (kernel/camera.cu:43-54):
+// count of steps inside and the first one.  The inside set is contiguous
+// (each coordinate of a step is monotone in j), so the walk stops at the
+// first outside step after the inside run.
+__host__ __device__ __forceinline__ uint32_t cut_walk(f3 n, f3 dir, float interval, uint32_t cand,
+                                                      f3 &ori) {
+    uint32_t count = 0;
+    ori = f3{0.0f, 0.0f, 0.0f};
+    for (uint32_t j = 0; j < cand; ++j) {
+        const f3 p = cut_step(n, dir, interval, j);
+        if (in_cube(p)) {
+            if (count == 0) ori = p;
+            ++count;
+        } else if (count != 0) {
+            break;
+        }
+    }
+    return count;
+}
+
+// The same count and first inside step without walking from the near plane:
+// a float64 slab test gives the real-valued inside range (lo, hi) of j, and
+// the exact float32 predicate is evaluated on the steps within 2 of its ends
+// only.  A float32 step lies within 5e-7 of its real value (three roundings
+// of magnitudes below 4), so no inside step lies outside
+// [floor(lo) - 2, ceil(hi) + 2] as long as every axis moves at least 1e-4
+// per step (the error is then under 0.005 steps); a ray that grazes a face
+// more slowly takes the walk.  Bit-identical to cut_walk.
+__host__ __device__ __forceinline__ uint32_t cut_closed_form(f3 n, f3 dir, float interval,
+                                                             uint32_t cand, f3 &ori) {
+    ori = f3{0.0f, 0.0f, 0.0f};
+    if (cand == 0) return 0;
+    const double b = (double)0.9999f;
+    double lo = 0.0, hi = (double)cand - 1.0;
+    const float na[3] = {n.x, n.y, n.z}, da[3] = {dir.x, dir.y, dir.z};
+    for (int a = 0; a < 3; ++a) {
+        if (da[a] == 0.0f) {  // the coordinate stays exactly n: inside for every j, or none
+            if (!(na[a] < 0.9999f && na[a] > -0.9999f)) return 0;
+            continue;
+        }
+        const double step = (double)da[a] * (double)interval;
+        if (step < 1e-4 && step > -1e-4) return cut_walk(n, dir, interval, cand, ori);
+        const double t1 = (-b - (double)na[a]) / step, t2 = (b - (double)na[a]) / step;
+        lo = t1 < t2 ? (t1 > lo ? t1 : lo) : (t2 > lo ? t2 : lo);
+        hi = t1 < t2 ? (t2 < hi ? t2 : hi) : (t1 < hi ? t1 : hi);
+    }
+    if (lo > hi + 4.0) return 0;  // no step within 2 of an inside real j
+    const double flo = lo - 2.0 < 0.0 ? 0.0 : lo - 2.0;
+    const double fhi = hi + 2.0 > (double)cand - 1.0 ? (double)cand - 1.0 : hi + 2.0;
+    if (fhi < flo) return 0;
+    uint32_t j0 = (uint32_t)flo;                  // floor (non-negative)
+    uint32_t j1 = (uint32_t)fhi;
+    if ((double)j1 < fhi) ++j1;                   // ceil
+    if (j1 > cand - 1) j1 = cand - 1;
+    while (j0 <= j1 && !in_cube(cut_step(n, dir, interval, j0))) ++j0;
+    if (j0 > j1) return 0;
+    while (j1 > j0 && !in_cube(cut_step(n, dir, interval, j1))) --j1;
+    ori = cut_step(n, dir, interval, j0);
+    return j1 - j0 + 1;
+}
+
 // Ray of pixel `pix` (row-major) and its in-cube step count / first inside
-// step (kernel/camera.cu:12-58).  The candidate loop is walked exactly as the
-// reference walks it; the inside set is contiguous (each coordinate of
-// near + (dir*j)*interval is monotone in j), so the walk stops at the first
-// outside step after the inside run.
-__device__ __forceinline__ RayCutResult ray_cut_pixel(const qsdvr_camera &c, uint32_t pix,
-                                                      float interval) {
+// step (kernel/camera.cu:12-58; the count from cut_closed_form).
+__host__ __device__ __forceinline__ RayCutResult ray_cut_pixel(const qsdvr_camera &c, uint32_t pix,
+                                                               float interval) {
 #pragma clang fp contract(off)
     float u = (float)(pix % c.width) + 0.5f;
     float v = (float)(pix / c.width) + 0.5f;
@@ -58,19 +124,8 @@ __device__ __forceinline__ RayCutResult ray_cut_pixel(const qsdvr_camera &c, uin
     else
         end = (f.z - n.z) / dir.z;
     uint32_t cand = (uint32_t)(end / interval);
-    uint32_t count = 0;
-    f3 ori{0.0f, 0.0f, 0.0f};
-    for (uint32_t j = 0; j < cand; ++j) {
-        float fj = (float)j;
-        f3 p{n.x + (dir.x * fj) * interval, n.y + (dir.y * fj) * interval,
-             n.z + (dir.z * fj) * interval};
-        if (in_cube(p)) {
-            if (count == 0) ori = p;
-            ++count;
-        } else if (count != 0) {
-            break;
-        }
-    }
+    f3 ori;
+    const uint32_t count = cut_closed_form(n, dir, interval, cand, ori);
     return RayCutResult{dir, ori, count};
 }
 
