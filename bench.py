@@ -305,17 +305,20 @@ def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps
                                        None, w.interval, s, args.scatter)
         ((c - targets[p]) ** 2).mean().backward()
 
-    def guided(p):
+    def guided(p, gamma=proposal.SAMPLE_GAMMA):
         G.grad = None
         D.grad = None
         c, _ = proposal.render_proposal(G, D, cam_buf[p * CB:(p + 1) * CB], 1, w.width, w.height,
-                                        w.interval, s, prop, K=K)
+                                        w.interval, s, prop, K=K, gamma=gamma)
         ((c - targets[p]) ** 2).mean().backward()
 
     out = {"logistic_coef": s, "fine_samples": K, "coarse_bins": 64,
            "sampler_eps": proposal.SAMPLE_EPS, "sampler_gamma": proposal.SAMPLE_GAMMA,
            "train_steps": args.proposal_train_steps, "train_seconds": round(train_s, 2)}
-    for name, fn in (("plain", plain), ("proposal", guided)):
+    runs = [("plain", plain), ("proposal", guided)]
+    for g in extra_gammas:  # sharper samplers: same kernels, timed as well
+        runs.append((f"proposal_gamma{g:g}", lambda p, g=g: guided(p, g)))
+    for name, fn in runs:
         for i in range(2):
             fn(n_train + i)
         torch.cuda.synchronize()
@@ -329,6 +332,9 @@ def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps
         out[name + "_ms_per_step"] = round(ms, 3)
         out[name + "_rays_per_s"] = round(P / (ms / 1e3), 1)
     out["speedup"] = round(out["plain_ms_per_step"] / out["proposal_ms_per_step"], 3)
+    for g in extra_gammas:
+        out[f"speedup_gamma{g:g}"] = round(out["plain_ms_per_step"] /
+                                           out[f"proposal_gamma{g:g}_ms_per_step"], 3)
     with torch.no_grad():
         test = cams[n - 5:]
         tb = fused.camera_buffer(test)
@@ -340,7 +346,7 @@ def measure_proposal(w, grid, data, cams, cam_buf, targets, s, args, K=32, steps
         out["psnr_proposal_vs_gt_db"] = round(proposal.psnr(pr, gt), 3)
         out["psnr_loss_db"] = round(out["psnr_plain_vs_gt_db"] - out["psnr_proposal_vs_gt_db"], 3)
         out["psnr_proposal_vs_plain_db"] = round(proposal.psnr(pr, pl), 3)
-        for g in extra_gammas:  # sharper samplers (same cost, PSNR only)
+        for g in extra_gammas:
             pg, _ = proposal.render_proposal(grid, data, tb, len(test), w.width, w.height,
                                              w.interval, s, prop, K=K, gamma=g)
             out[f"psnr_loss_db_gamma{g:g}"] = round(out["psnr_plain_vs_gt_db"] -

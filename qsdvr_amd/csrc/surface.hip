@@ -162,20 +162,42 @@ __global__ void __launch_bounds__(kL2GBlock) layer_grad_finish_kernel(
 }
 
 // --------------------------------------------------- SDF and normals ----
+// SampleSDF / SampleNormal forward: SPL samples per lane, lane-strided so
+// that every load instruction stays coalesced; the indices of all SPL
+// samples are loaded first, then their quadrics and points, so a wave makes
+// two memory round trips for SPL samples.
+constexpr int kSampleSPL = 4;
+
 __global__ void __launch_bounds__(256) sample_sdf_fwd_kernel(float *__restrict__ out,
                                                              const float *__restrict__ grid,
                                                              const float *__restrict__ pts,
                                                              const int32_t *__restrict__ idx,
                                                              int64_t n, int64_t cells,
                                                              float inv_reso) {
-    int64_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int64_t c = idx[i];
-    if (c < 0 || c >= cells) {
-        out[i] = __builtin_nanf("");
-        return;
+    const int64_t base = xcd_block() * blockDim.x * kSampleSPL + threadIdx.x;
+    int64_t c[kSampleSPL];
+#pragma unroll
+    for (int u = 0; u < kSampleSPL; ++u) {
+        const int64_t i = base + (int64_t)u * blockDim.x;
+        c[u] = i < n ? (int64_t)idx[i] : -1;
     }
-    out[i] = quadric_sdf(load_quadric(grid, c), load3(pts, i), inv_reso);
+    Quadric q[kSampleSPL];
+    f3 p[kSampleSPL];
+#pragma unroll
+    for (int u = 0; u < kSampleSPL; ++u) {
+        const int64_t i = base + (int64_t)u * blockDim.x;
+        if (c[u] >= 0 && c[u] < cells) {
+            q[u] = load_quadric(grid, c[u]);
+            p[u] = load3(pts, i);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kSampleSPL; ++u) {
+        const int64_t i = base + (int64_t)u * blockDim.x;
+        if (i >= n) continue;
+        out[i] = (c[u] < 0 || c[u] >= cells) ? __builtin_nanf("")
+                                             : quadric_sdf(q[u], p[u], inv_reso);
+    }
 }
 
 __global__ void __launch_bounds__(256) sample_sdf_bwd_kernel(const float *__restrict__ up,
@@ -200,17 +222,36 @@ __global__ void __launch_bounds__(256) sample_normal_fwd_kernel(float *__restric
                                                                 const float *__restrict__ pts,
                                                                 const int32_t *__restrict__ idx,
                                                                 int64_t n, int64_t cells) {
-    int64_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int64_t c = idx[i];
-    f3 r;
-    if (c < 0 || c >= cells) {
-        float q = __builtin_nanf("");
-        r = f3{q, q, q};
-    } else {
-        r = quadric_normal(load_quadric(grid, c), load3(pts, i));
+    const int64_t base = xcd_block() * blockDim.x * kSampleSPL + threadIdx.x;
+    int64_t c[kSampleSPL];
+#pragma unroll
+    for (int u = 0; u < kSampleSPL; ++u) {
+        const int64_t i = base + (int64_t)u * blockDim.x;
+        c[u] = i < n ? (int64_t)idx[i] : -1;
     }
-    store3(out, i, r);
+    Quadric q[kSampleSPL];
+    f3 p[kSampleSPL];
+#pragma unroll
+    for (int u = 0; u < kSampleSPL; ++u) {
+        const int64_t i = base + (int64_t)u * blockDim.x;
+        if (c[u] >= 0 && c[u] < cells) {
+            q[u] = load_quadric(grid, c[u]);
+            p[u] = load3(pts, i);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kSampleSPL; ++u) {
+        const int64_t i = base + (int64_t)u * blockDim.x;
+        if (i >= n) continue;
+        f3 r;
+        if (c[u] < 0 || c[u] >= cells) {
+            const float nan = __builtin_nanf("");
+            r = f3{nan, nan, nan};
+        } else {
+            r = quadric_normal(q[u], p[u]);
+        }
+        store3(out, i, r);
+    }
 }
 
 __global__ void __launch_bounds__(256) sample_normal_bwd_kernel(const float *__restrict__ up,
@@ -323,7 +364,7 @@ int qsdvr_sample_sdf_forward(float *out, const float *sdf_grid, const float *poi
     QSDVR_CHECK_SAMPLES("qsdvr_sample_sdf_forward");
     QSDVR_REQUIRE(out && sdf_grid && point && index, "qsdvr_sample_sdf_forward: null pointer");
     QSDVR_REQUIRE(reso != 0.0f, "qsdvr_sample_sdf_forward: reso must be nonzero");
-    sample_sdf_fwd_kernel<<<grid_1d(n, 256), 256, 0, as_stream(stream)>>>(
+    sample_sdf_fwd_kernel<<<grid_1d(n, 256 * kSampleSPL), 256, 0, as_stream(stream)>>>(
         out, sdf_grid, point, index, n, cell_count, 1.0f / reso);
     QSDVR_LAUNCHED();
     return QSDVR_OK;
@@ -347,7 +388,7 @@ int qsdvr_sample_normal_forward(float *out, const float *sdf_grid, const float *
                                 qsdvr_stream stream) {
     QSDVR_CHECK_SAMPLES("qsdvr_sample_normal_forward");
     QSDVR_REQUIRE(out && sdf_grid && point && index, "qsdvr_sample_normal_forward: null pointer");
-    sample_normal_fwd_kernel<<<grid_1d(n, 256), 256, 0, as_stream(stream)>>>(
+    sample_normal_fwd_kernel<<<grid_1d(n, 256 * kSampleSPL), 256, 0, as_stream(stream)>>>(
         out, sdf_grid, point, index, n, cell_count);
     QSDVR_LAUNCHED();
     return QSDVR_OK;
