@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("fwd+bwd rays/s at 128³ grid 800×800, plain vs proposal at "
           "≤0.1 dB PSNR loss")
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 
 
 def log(*a):
@@ -60,6 +61,16 @@ def parse():
                     help="run a plain device copy of this many bytes first (a counter "
                          "calibration kernel of known traffic for rocprofv3 --pmc runs)")
     return ap.parse_args()
+
+
+def load_profile_record(name):
+    """A JSON record of rocprofv3 counter passes kept in profiles/ (None if
+    absent)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def measured_traffic():
@@ -214,22 +225,43 @@ def main():
         log(f"large: {large}")
 
     # ------------------------------------------------------- roofline ----
+    # The dominant kernel against its two rooflines, with the per-launch counts
+    # from the rocprofv3 counter passes recorded in profiles/: HBM (measured
+    # FETCH/WRITE bytes) and float32 compute (executed VALU and MFMA float32
+    # operations).  `bound` is the roofline it runs closer to.  For "hbm",
+    # `achieved` is its algorithmic bytes per launch over this run's launch
+    # time; for "mfma" (float32 compute: matrix cores and VALU), its executed
+    # float32 FLOPs per launch over that time.
     dom = max(kern_ms, key=kern_ms.get)
+    t_launch = kern_ms[dom] / 1e3
     byts = algorithmic_bytes(w.R, 32, P) if dom == "fused_backward" else (
         28 * w.R ** 3 + 4 * 32 * (w.R + 1) ** 3 + 16 * P)
-    achieved = byts / (kern_ms[dom] / 1e3) / 1e9
-    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": None, "algorithmic_bytes_per_launch": byts,
-                "avg_launch_ms": round(kern_ms[dom], 4)}
     traffic = measured_traffic()
+    util = measured_utilization() or {}
+    flops = (util.get(dom) or {}).get("fp32_flops_per_launch")
+    traffic_frac = None
     if dom == "fused_backward" and traffic is not None:
-        roofline["traffic"] = traffic["bytes_per_launch"]
+        traffic_frac = traffic["bytes_per_launch"] / t_launch / 1e9 / HBM_PEAK_GBS
+    compute_frac = flops / t_launch / 1e12 / FP32_PEAK_TFLOPS if flops else None
+    bound = "mfma" if (compute_frac is not None and traffic_frac is not None
+                       and compute_frac > traffic_frac) else "hbm"
+    if bound == "hbm":
+        achieved, peak, unit = byts / t_launch / 1e9, HBM_PEAK_GBS, "GB/s"
+    else:
+        achieved, peak, unit = flops / t_launch / 1e12, FP32_PEAK_TFLOPS, "TFLOP/s"
+    roofline = {"kernel": dom, "bound": bound, "achieved": round(achieved, 2), "peak": peak,
+                "unit": unit, "frac": round(achieved / peak, 5),
+                "traffic": traffic["bytes_per_launch"] if traffic_frac is not None else None,
+                "traffic_frac": round(traffic_frac, 5) if traffic_frac is not None else None,
+                "algorithmic_bytes_per_launch": byts,
+                "algorithmic_hbm_frac": round(byts / t_launch / 1e9 / HBM_PEAK_GBS, 5),
+                "fp32_flops_per_launch": flops,
+                "fp32_frac": round(compute_frac, 5) if compute_frac is not None else None,
+                "avg_launch_ms": round(kern_ms[dom], 4)}
+    if traffic_frac is not None:
         roofline["traffic_source"] = traffic["source"]
-        # measured HBM bytes over this run's launch time: the kernel's real
-        # DRAM rate, against the same peak
-        roofline["traffic_frac"] = round(traffic["bytes_per_launch"] / (kern_ms[dom] / 1e3)
-                                         / 1e9 / HBM_PEAK_GBS, 5)
+    if flops:
+        roofline["flops_source"] = util.get("source")
 
     # ------------------------------------------ op-surface HBM fraction ----
     # The op-surface path (the reference's 14 ops, each reading its inputs
@@ -247,6 +279,14 @@ def main():
                       "algorithmic_bytes_per_ray": round(b_op, 1),
                       "achieved_GBs": round(b_op * op_rays / 1e9, 1),
                       "frac": round(b_op * op_rays / 1e9 / HBM_PEAK_GBS, 4)}
+        # measured HBM bytes of one op-surface step (rocprofv3 FETCH/WRITE
+        # passes of all its kernels) over this run's step time
+        rec = load_profile_record("op_surface_traffic.json")
+        if rec is not None:
+            op_surface["traffic_bytes_per_step"] = rec["bytes_per_step"]
+            op_surface["traffic_GBs"] = round(rec["bytes_per_step"] / (op_ms / 1e3) / 1e9, 1)
+            op_surface["traffic_frac"] = round(op_surface["traffic_GBs"] / HBM_PEAK_GBS, 4)
+            op_surface["traffic_source"] = rec["source"]
 
     # --------------------------------------------------- CPU baseline ----
     cpu = None
