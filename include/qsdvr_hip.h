@@ -267,6 +267,18 @@ int qsdvr_sample_normal_backward_sorted(const float *grad_out, const float *sdf_
                                         float *grad_grid, int64_t n,
                                         int64_t cell_count, void *workspace,
                                         size_t workspace_bytes, qsdvr_stream stream);
+/* SampleNormalBackward (kernel/surface.cu:369-375, bound at kernel/qsdvr.cpp:22)
+ * on the sort a preceding qsdvr_grid_interpolation_backward_sorted call made
+ * of the same index list: `workspace` is that call's workspace (n samples,
+ * cell resolution reso = vertex_reso - 1, even), whose (parity class, cell)
+ * order groups equal cells as the cell order does.  Same sums as
+ * qsdvr_sample_normal_backward_sorted, without its own sort; the two calls
+ * must run on one stream, and the workspace must not change in between. */
+int qsdvr_sample_normal_backward_presorted(const float *grad_out, const float *sdf_grid,
+                                           const float *point, const int32_t *index,
+                                           float *grad_grid, int64_t n, int32_t reso,
+                                           void *workspace, size_t workspace_bytes,
+                                           qsdvr_stream stream);
 
 /* ------------------------------------------------------------- fused --- */
 

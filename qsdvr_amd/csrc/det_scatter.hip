@@ -217,9 +217,12 @@ struct NormalSamples {
 // wave where such a run starts adds its tail sum and the head sums of the
 // waves the run covers, in order, to the cell.  The same input therefore
 // always gives the same bits.
+// Keys are cells, or carry bits above the cell that `cmask` clears (the
+// (parity class, cell) keys of GridInterpolationBackward's sort).
 template <class Src>
 __global__ void __launch_bounds__(256) quadric_seg_kernel(const uint32_t *__restrict__ key,
                                                           Src src, int64_t n, uint32_t invalid,
+                                                          uint32_t cmask,
                                                           float *__restrict__ gq,
                                                           float *__restrict__ head,
                                                           float *__restrict__ tail) {
@@ -230,7 +233,7 @@ __global__ void __launch_bounds__(256) quadric_seg_kernel(const uint32_t *__rest
     const int64_t p = p0 + lane;
     const uint32_t k = p < n ? key[p] : invalid;
     float v[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (k != invalid) src.add(p, k, v);
+    if (k != invalid) src.add(p, k & cmask, v);
     const uint32_t kp = __shfl_up(k, 1, kWave), kn = __shfl_down(k, 1, kWave);
     const bool start = lane == 0 || kp != k;
     const bool last = lane == kWave - 1 || kn != k;
@@ -262,7 +265,7 @@ __global__ void __launch_bounds__(256) quadric_seg_kernel(const uint32_t *__rest
 #pragma unroll
             for (int j = 0; j < 7; ++j) dst[j] = v[j];
         } else {
-            float *g = gq + 7 * (int64_t)k;
+            float *g = gq + 7 * (int64_t)(k & cmask);
 #pragma unroll
             for (int j = 0; j < 7; ++j) g[j] += v[j];
         }
@@ -272,6 +275,7 @@ __global__ void __launch_bounds__(256) quadric_seg_kernel(const uint32_t *__rest
 // Pass 2: one thread per wave of pass 1.
 __global__ void __launch_bounds__(256) quadric_seg_join_kernel(const uint32_t *__restrict__ key,
                                                                int64_t n, uint32_t invalid,
+                                                               uint32_t cmask,
                                                                const float *__restrict__ head,
                                                                const float *__restrict__ tail,
                                                                float *__restrict__ gq) {
@@ -290,7 +294,7 @@ __global__ void __launch_bounds__(256) quadric_seg_join_kernel(const uint32_t *_
         const int64_t q = (u + 1) * kWave;
         if (q >= n || key[q] != k) break;
     }
-    float *g = gq + 7 * (int64_t)k;
+    float *g = gq + 7 * (int64_t)(k & cmask);
 #pragma unroll
     for (int j = 0; j < 7; ++j) g[j] += t[j];
 }
@@ -298,12 +302,14 @@ __global__ void __launch_bounds__(256) quadric_seg_join_kernel(const uint32_t *_
 // Both passes over n sorted records; head and tail hold 7 floats per wave.
 template <class Src>
 int quadric_seg_sum(const uint32_t *key, Src src, int64_t n, uint32_t invalid, float *gq,
-                    float *head, float *tail, hipStream_t s) {
+                    float *head, float *tail, hipStream_t s, uint32_t cmask = 0xFFFFFFFFu) {
     if (n == 0) return QSDVR_OK;
-    quadric_seg_kernel<<<grid_1d(n, 256), 256, 0, s>>>(key, src, n, invalid, gq, head, tail);
+    quadric_seg_kernel<<<grid_1d(n, 256), 256, 0, s>>>(key, src, n, invalid, cmask, gq, head,
+                                                       tail);
     QSDVR_LAUNCHED();
     const int64_t waves = (n + kWave - 1) / kWave;
-    quadric_seg_join_kernel<<<grid_1d(waves, 256), 256, 0, s>>>(key, n, invalid, head, tail, gq);
+    quadric_seg_join_kernel<<<grid_1d(waves, 256), 256, 0, s>>>(key, n, invalid, cmask, head,
+                                                                tail, gq);
     QSDVR_LAUNCHED();
     return QSDVR_OK;
 }
@@ -846,6 +852,29 @@ int qsdvr_sample_normal_backward_sorted(const float *grad_out, const float *sdf_
                                         (unsigned)bits_above((uint64_t)cell_count), s));
     return quadric_seg_sum(b.key_s, NormalSamples{val_s, sdf_grid, point, grad_out}, n,
                            (uint32_t)cell_count, grad_grid, b.head, b.tail, s);
+}
+
+int qsdvr_sample_normal_backward_presorted(const float *grad_out, const float *sdf_grid,
+                                           const float *point, const int32_t *index,
+                                           float *grad_grid, int64_t n, int32_t reso,
+                                           void *workspace, size_t workspace_bytes,
+                                           qsdvr_stream stream) {
+    QSDVR_REQUIRE(n >= 0 && reso >= 2 && reso % 2 == 0,
+                  "qsdvr_sample_normal_backward_presorted: bad sizes (even reso)");
+    if (n == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(grad_out && sdf_grid && point && index && grad_grid,
+                  "qsdvr_sample_normal_backward_presorted: null pointer");
+    const int64_t cells = (int64_t)reso * reso * reso;
+    const int cbits = bits_at_least((uint64_t)cells);
+    QSDVR_REQUIRE(cbits + 4 <= 32, "qsdvr_sample_normal_backward_presorted: reso too large");
+    OpBuffers b;
+    int st = op_prepare(workspace, workspace_bytes, n, cells, b,
+                        "qsdvr_sample_normal_backward_presorted");
+    if (st != QSDVR_OK) return st;
+    const uint32_t *val_s = reinterpret_cast<const uint32_t *>(b.val_s);
+    return quadric_seg_sum(b.key_s, NormalSamples{val_s, sdf_grid, point, grad_out}, n,
+                           8u << cbits, grad_grid, b.head, b.tail, as_stream(stream),
+                           (1u << cbits) - 1u);
 }
 
 }  // extern "C"

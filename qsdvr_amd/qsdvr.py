@@ -132,7 +132,25 @@ def _channels(grid):
     return int(grid.shape[-1])
 
 
+# Every drop-in op counts its calls, so that GridInterpolationBackward's
+# sort is reused only by a SampleNormalBackward that immediately follows it
+# (see _remember_render_sort).
+_calls = 0
+
+
+def _op(fn):
+    import functools
+
+    @functools.wraps(fn)
+    def wrapped(*args, **kwargs):
+        global _calls
+        _calls += 1
+        return fn(*args, **kwargs)
+    return wrapped
+
+
 # ------------------------------------------------------ render ops ----
+@_op
 def GridInterpolationForward(dataGrid, pointList, indexList, out, reso):
     """kernel/render.cu:317-323; oracle semantics of render_torch.py:128-143.
     ``reso`` is dataGrid.size(0) = R+1, as models/render.py:15 passes it."""
@@ -148,6 +166,7 @@ def GridInterpolationForward(dataGrid, pointList, indexList, out, reso):
               _ptr(indexList), _ptr(out), n, R1, Cc, _stream())
 
 
+@_op
 def GridInterpolationBackward(dataGrid, pointList, indexList, out, reso):
     """kernel/render.cu:325-331.  Roles as in the reference: ``dataGrid`` is
     the zeroed gradient accumulator, ``out`` the upstream gradient."""
@@ -164,11 +183,13 @@ def GridInterpolationBackward(dataGrid, pointList, indexList, out, reso):
         ws = _workspace(ws_n)
         _lib.call("qsdvr_grid_interpolation_backward_sorted", _ptr(dataGrid), _ptr(pointList),
                   _ptr(indexList), _ptr(out), n, R1, Cc, _ptr(ws), ws_n, _stream())
+        _remember_render_sort(indexList, n, R1 - 1, ws, ws_n)
         return
     _lib.call("qsdvr_grid_interpolation_backward", _ptr(dataGrid), _ptr(pointList),
               _ptr(indexList), _ptr(out), n, R1, Cc, _stream())
 
 
+@_op
 def ShaderForward(out, normalList, viewDirList, dataList, specularList):
     """kernel/render.cu:333-339; render_torch.py:144-152."""
     n = _points("normalList", normalList).shape[0]
@@ -180,6 +201,7 @@ def ShaderForward(out, normalList, viewDirList, dataList, specularList):
               _ptr(dataList), _ptr(specularList), n, int(dataList.shape[1]), _stream())
 
 
+@_op
 def ShaderBackward(out, normalList, viewDirList, dataList, specularList, normalGradList,
                    dataGradList):
     """kernel/render.cu:341-347.  ``out`` is the upstream gradient."""
@@ -199,6 +221,7 @@ def _raylist(rayList):
     return _t("rayList", rayList, torch.int32, ndim=2, shape=(None, 4)).shape[0]
 
 
+@_op
 def RayAggregateForward(out, rgbList, sdfList, rayList, alphaList, TList, logisticCoef):
     """kernel/render.cu:349-355; render_torch.py:155-167."""
     rays = _raylist(rayList)
@@ -212,6 +235,7 @@ def RayAggregateForward(out, rgbList, sdfList, rayList, alphaList, TList, logist
               _stream())
 
 
+@_op
 def RayAggregateBackward(outgrad, rgbList, sdfList, rayList, alphaList, TList, rgbGradList,
                          sdfGradList, logisticCoef):
     """kernel/render.cu:357-363."""
@@ -237,6 +261,7 @@ def _grid7(name, g):
     return g.numel() // 7
 
 
+@_op
 def SampleSDFForward(out, sdfGrid, pointList, indexList, reso):
     """kernel/surface.cu:343-350; render_torch.py:77-90."""
     cells = _grid7("sdfGrid", sdfGrid)
@@ -249,6 +274,7 @@ def SampleSDFForward(out, sdfGrid, pointList, indexList, reso):
               _ptr(indexList), n, cells, float(reso), _stream())
 
 
+@_op
 def SampleSDFBackward(out, sdfGrid, pointList, indexList, sdfGradGrid, reso):
     """kernel/surface.cu:352-359.  ``out`` is the upstream gradient."""
     cells = _grid7("sdfGrid", sdfGrid)
@@ -271,6 +297,7 @@ def SampleSDFBackward(out, sdfGrid, pointList, indexList, sdfGradGrid, reso):
               _ptr(indexList), _ptr(sdfGradGrid), n, cells, float(reso), _stream())
 
 
+@_op
 def SampleNormalForward(out, sdfGrid, pointList, indexList):
     """kernel/surface.cu:361-367; render_torch.py:91-100."""
     cells = _grid7("sdfGrid", sdfGrid)
@@ -281,6 +308,7 @@ def SampleNormalForward(out, sdfGrid, pointList, indexList):
               _ptr(indexList), n, cells, _stream())
 
 
+@_op
 def SampleNormalBackward(out, sdfGrid, pointList, indexList, sdfGradGrid):
     """kernel/surface.cu:369-375.  ``out`` is the upstream gradient."""
     cells = _grid7("sdfGrid", sdfGrid)
@@ -291,6 +319,13 @@ def SampleNormalBackward(out, sdfGrid, pointList, indexList, sdfGradGrid):
     if sdfGradGrid.numel() != sdfGrid.numel():
         raise ValueError("sdfGradGrid must match sdfGrid")
     if quadric_backward_path(n, sdfGrid) == "sorted":
+        reuse = _take_render_sort(indexList, n, cells)
+        if reuse is not None:  # GridInterpolationBackward already sorted these indices
+            R, ws, ws_n = reuse
+            _lib.call("qsdvr_sample_normal_backward_presorted", _ptr(out), _ptr(sdfGrid),
+                      _ptr(pointList), _ptr(indexList), _ptr(sdfGradGrid), n, R, _ptr(ws),
+                      ws_n, _stream())
+            return
         ws_n = _sorted_workspace(n, cells)
         ws = _workspace(ws_n)
         _lib.call("qsdvr_sample_normal_backward_sorted", _ptr(out), _ptr(sdfGrid),
@@ -362,6 +397,42 @@ def quadric_backward_path(n, sdfGrid):
     return "sorted" if ok else "atomic"
 
 
+# The reference's backward (models/render.py) runs GridInterpolationBackward
+# and then SampleNormalBackward on the same render index list.  The sorted
+# GridInterpolationBackward orders those indices by (parity class, cell),
+# which also groups equal cells, so SampleNormalBackward can sum its runs on
+# that order instead of sorting again.  The workspace holding it is kept
+# here (one entry) with a weak reference to the index tensor, its version
+# counter (an in-place change invalidates it), its size and resolution, the
+# stream and the op-call count: only the very next op call, if it is
+# SampleNormalBackward on that tensor, takes it.  (The reused order sums a
+# cell's samples in a different tree than SampleNormalBackward's own sort,
+# so the bits then depend on the preceding call -- the same sequence of
+# calls still gives the same bits.)
+_render_sort = None
+
+
+def _remember_render_sort(index, n, R, ws, ws_n):
+    global _render_sort
+    import weakref
+    _render_sort = (weakref.ref(index), index._version, int(n), int(R), ws, int(ws_n),
+                    torch.cuda.current_stream().cuda_stream, _calls)
+
+
+def _take_render_sort(index, n, cells):
+    """(R, workspace, bytes) of a matching GridInterpolationBackward sort, or
+    None; the entry is consumed."""
+    global _render_sort
+    ent, _render_sort = _render_sort, None
+    if ent is None:
+        return None
+    ref, version, n0, R, ws, ws_n, stream, call = ent
+    if (call + 1 != _calls or ref() is not index or index._version != version or n0 != int(n)
+            or R ** 3 != int(cells) or stream != torch.cuda.current_stream().cuda_stream):
+        return None
+    return R, ws, ws_n
+
+
 def _sorted_workspace(n, cells):
     nb = int(_lib.fn("qsdvr_sorted_scatter_workspace_size")(int(n), int(cells)))
     if nb == 0:
@@ -369,6 +440,7 @@ def _sorted_workspace(n, cells):
     return nb
 
 
+@_op
 def LayerToGridForward(out, xLayer, yLayer, zLayer, offset):
     """kernel/surface.cu:148-229; render_torch.py:30-75.  Layers may be CPU
     tensors (as in models/render.py:177-183) or CUDA tensors."""
@@ -382,6 +454,7 @@ def LayerToGridForward(out, xLayer, yLayer, zLayer, offset):
               _ptr(zLayer), _ptr(offset), R, 1 if dev == "cpu" else 0, _ptr(ws), ws_n, _stream())
 
 
+@_op
 def LayerToGridBackward(out, xLayer, yLayer, zLayer, offset):
     """kernel/surface.cu:231-341.  ``out`` is the upstream gradient grid; the
     layer tensors and offset are OUTPUTS (CPU as in models/render.py:117-120,
@@ -398,6 +471,7 @@ def LayerToGridBackward(out, xLayer, yLayer, zLayer, offset):
 
 
 # ------------------------------------------------------- camera ops ----
+@_op
 def RayCut(args: CameraArgs, mask, rayList, originList, dirList, interval):
     """kernel/camera.cu:98-130.  ``mask``: bool [pixelCount] (CPU as in the
     reference; a CUDA mask is copied to the host).  Sets ``args.sdfCount``
@@ -429,6 +503,7 @@ def RayCut(args: CameraArgs, mask, rayList, originList, dirList, interval):
     args.renderCount = int(rt.value)
 
 
+@_op
 def GenerateRayPoints(args: CameraArgs, sdfPointList, sdfIndexList, renderPointList,
                       renderIndexList, viewDirList, rayList, originList, dirList, interval,
                       baseVolumeReso):

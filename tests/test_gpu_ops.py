@@ -271,3 +271,57 @@ def test_sorted_interpolation_backward_matches_oracle(monkeypatch, C):
     qsdvr.GridInterpolationBackward(gd, pts.cuda(), idx.cuda(), up.cuda(), R + 1)
     torch.cuda.synchronize()
     assert_close(gd, d64.grad, BWD, f"sorted GridInterpolationBackward (C={C})")
+
+
+def test_normal_backward_reuses_the_interpolation_sort(monkeypatch):
+    """On a full 800x800 pose: SampleNormalBackward right after
+    GridInterpolationBackward on the same render indices sums its runs on
+    that call's (parity class, cell) order instead of sorting again; it
+    equals SampleNormalBackward's own sorted sums to 1e-4, repeats bitwise,
+    and is not used when another call intervenes or the indices changed."""
+    from qsdvr_amd import _lib
+    from qsdvr_amd.camera import Camera
+    w = scenes.WORKLOADS["plain"]
+    cam = scenes.orbit_cameras(100, w.width, w.height)[23]
+    grid, data, _ = scenes.build_grids(w, seed=0, device="cuda")
+    a = qsdvr.CameraArgs.from_camera(cam)
+    cc = Camera(a, w.interval, w.R)
+    cc.GenerateRayPoints(None)
+    inp = cc.GetRayPoints()
+    rp, ri = inp["renderPointList"], inp["renderIndexList"]
+    g = torch.Generator(device="cuda").manual_seed(3)
+    up_n = torch.randn((rp.shape[0], 3), generator=g, device="cuda")
+    up_d = torch.randn((rp.shape[0], 32), generator=g, device="cuda")
+    names = []
+    real_call = _lib.call
+    monkeypatch.setattr(_lib, "call", lambda name, *a: (names.append(name), real_call(name, *a)))
+    monkeypatch.setattr(qsdvr, "SCATTER", "sorted")
+
+    def normal():
+        gn = torch.zeros_like(grid)
+        qsdvr.SampleNormalBackward(up_n, grid, rp, ri, gn)
+        return gn
+
+    def interp():
+        gd = torch.zeros_like(data)
+        qsdvr.GridInterpolationBackward(gd, rp, ri, up_d, w.R + 1)
+
+    own = normal()
+    interp()
+    reused = normal()
+    interp()
+    reused2 = normal()
+    assert names.count("qsdvr_sample_normal_backward_presorted") == 2
+    assert torch.equal(reused, reused2)
+    assert_close(reused.reshape(-1, 7), own.reshape(-1, 7), BWD, "presorted vs own sort")
+    names.clear()
+    interp()
+    qsdvr.SampleNormalForward(torch.empty_like(up_n), grid, rp, ri)   # an intervening call
+    again = normal()
+    assert "qsdvr_sample_normal_backward_presorted" not in names
+    assert torch.equal(again, own)
+    names.clear()
+    interp()
+    ri.add_(0)                                                        # in-place: new version
+    assert torch.equal(normal(), own)
+    assert "qsdvr_sample_normal_backward_presorted" not in names
