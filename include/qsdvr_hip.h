@@ -239,10 +239,12 @@ int qsdvr_ray_aggregate_backward(const float *grad_out, const float *rgb,
  * qsdvr_sample_sdf_backward (kernel/surface.cu:352-359) and
  * qsdvr_sample_normal_backward (kernel/surface.cu:369-375).  The samples
  * are radix-sorted by cell (SampleSDF sorts its point and upstream gradient
- * with the keys, the others sample indices) and every cell's samples are
- * summed in a fixed order before one read-modify-write of the cell's rows
- * (the appearance in 8 cell-parity passes), instead of one float atomic
- * per sample and coefficient: bitwise reproducible.  Workspace:
+ * with the keys, GridInterpolation its point and sample index, SampleNormal
+ * sample indices; GridInterpolation's keys put the cells in 8 parity classes)
+ * and every cell's samples are summed in a fixed order before one
+ * read-modify-write of the cell's rows (the appearance in 8 cell-parity
+ * passes), instead of one float atomic per sample and coefficient: bitwise
+ * reproducible.  Workspace:
  * qsdvr_sorted_scatter_workspace_size(n, cells) bytes (about 41 B per
  * sample plus the sort's temporary storage) with cells = R^3
  * (cell_count for the quadric ops, (vertex_reso - 1)^3 for the
@@ -269,13 +271,15 @@ int qsdvr_sample_normal_backward_sorted(const float *grad_out, const float *sdf_
                                         size_t workspace_bytes, qsdvr_stream stream);
 /* SampleNormalBackward (kernel/surface.cu:369-375, bound at kernel/qsdvr.cpp:22)
  * on the sort a preceding qsdvr_grid_interpolation_backward_sorted call made
- * of the same index list: `workspace` is that call's workspace (n samples,
- * cell resolution reso = vertex_reso - 1, even), whose (parity class, cell)
- * order groups equal cells as the cell order does.  Same sums as
- * qsdvr_sample_normal_backward_sorted, without its own sort; the two calls
- * must run on one stream, and the workspace must not change in between. */
+ * of the same point and index lists: `workspace` is that call's workspace
+ * (n samples, cell resolution reso = vertex_reso - 1, even), which holds
+ * the samples' points and indices in (parity class, cell) order -- equal
+ * cells grouped as the cell order groups them.  The points come from there,
+ * so the reference's pointList and indexList arguments are not taken.  Same
+ * sums as qsdvr_sample_normal_backward_sorted, without its own sort; the
+ * two calls must run on one stream, and the workspace must not change in
+ * between. */
 int qsdvr_sample_normal_backward_presorted(const float *grad_out, const float *sdf_grid,
-                                           const float *point, const int32_t *index,
                                            float *grad_grid, int64_t n, int32_t reso,
                                            void *workspace, size_t workspace_bytes,
                                            qsdvr_stream stream);

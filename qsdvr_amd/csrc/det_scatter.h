@@ -34,6 +34,38 @@ inline int det_cell_bits(int R) {
     return b;
 }
 
+// GridInterpolationBackward's sort keys.  A sample in cell (x, y, z) of an
+// R^3 grid (R even, h = R / 2) gets the key cls h^3 + x' + h y' + h^2 z', with
+// cls = (x & 1) | (y & 1) << 1 | (z & 1) << 2 the cell's parity class and
+// (x', y', z') = (x, y, z) >> 1; an invalid sample gets R^3.  The keys of one
+// class are contiguous and ordered as their cells, and a key needs no more
+// bits than the cell index (a class field above the cell bits would add
+// 3 bits and an invalid flag a fourth, one more radix pass at R = 128).
+struct ClassKeys {
+    uint32_t h, hh, h3;  // R / 2, (R / 2)^2, (R / 2)^3
+    int R;
+    __host__ __device__ __forceinline__ uint32_t key(int64_t x, int64_t y, int64_t z) const {
+        const uint32_t cls = (uint32_t)((x & 1) | ((y & 1) << 1) | ((z & 1) << 2));
+        return cls * h3 + (uint32_t)(x >> 1) + h * ((uint32_t)(y >> 1) + h * (uint32_t)(z >> 1));
+    }
+    __host__ __device__ __forceinline__ int64_t cell(uint32_t k) const {
+        uint32_t cls = 0;
+#pragma unroll
+        for (uint32_t c = 1; c < 8; ++c) cls += k >= c * h3 ? 1u : 0u;
+        uint32_t c = k - cls * h3;
+        const uint32_t z = c / hh;
+        c -= z * hh;
+        const uint32_t y = c / h, x = c - y * h;
+        return (int64_t)(2 * x + (cls & 1)) +
+               (int64_t)R * ((int64_t)(2 * y + ((cls >> 1) & 1)) + (int64_t)R * (2 * z + (cls >> 2)));
+    }
+};
+
+inline ClassKeys class_keys(int R) {
+    const uint32_t h = (uint32_t)(R / 2);
+    return ClassKeys{h, h * h, h * h * h, R};
+}
+
 struct DetBuffers {
     int64_t n_rays = 0, cap = 0;  // rays, sdf-sample slot capacity
     int C = 0, R = 0, cbits = 0;

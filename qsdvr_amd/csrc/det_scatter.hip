@@ -163,6 +163,18 @@ __device__ __forceinline__ int64_t run_end(const uint32_t *__restrict__ key, int
     }
 }
 
+// Sort keys of the quadric runs: the cell itself.
+struct CellKeys {
+    __device__ __forceinline__ int64_t cell(uint32_t k) const { return k; }
+};
+
+// What GridInterpolationBackward sorts with its keys: the sample's point and
+// its index (so that the run sums read the points in sorted order).
+struct alignas(16) PointRec {
+    float x, y, z;
+    uint32_t i;
+};
+
 // Where the record at sorted position `pos` of a quadric run gets its 7
 // coefficient gradients from: the adjoint's precomputed records (through
 // the sorted record indices), or (op surface) the closed forms of the
@@ -171,7 +183,7 @@ __device__ __forceinline__ int64_t run_end(const uint32_t *__restrict__ key, int
 struct QuadricRecords {
     const float4 *qval;
     const uint32_t *idx;
-    __device__ __forceinline__ void add(int64_t pos, uint32_t, float acc[7]) const {
+    __device__ __forceinline__ void add(int64_t pos, int64_t, float acc[7]) const {
         const int64_t r = idx[pos];
         const float4 a = qval[2 * r], c = qval[2 * r + 1];
         acc[0] += a.x;
@@ -189,7 +201,7 @@ struct SdfSamples {
     const float4 *pay;
     const float *grid;
     float inv_reso;
-    __device__ __forceinline__ void add(int64_t pos, uint32_t cell, float acc[7]) const {
+    __device__ __forceinline__ void add(int64_t pos, int64_t cell, float acc[7]) const {
         const float4 p = pay[pos];
         quadric_sdf_grad(load_quadric(grid, cell), f3{p.x, p.y, p.z}, p.w, inv_reso, acc);
     }
@@ -201,9 +213,22 @@ struct SdfSamples {
 struct NormalSamples {
     const uint32_t *idx;
     const float *grid, *pts, *up;
-    __device__ __forceinline__ void add(int64_t pos, uint32_t cell, float acc[7]) const {
+    __device__ __forceinline__ void add(int64_t pos, int64_t cell, float acc[7]) const {
         const int64_t r = idx[pos];
         quadric_normal_grad(load_quadric(grid, cell), load3(pts, r), load3(up, r), acc);
+    }
+};
+
+// SampleNormal record on GridInterpolationBackward's sort of the same
+// samples: the point in sorted order, the upstream gradient gathered
+// through the sample index sorted with it
+struct NormalPresorted {
+    const PointRec *pay;
+    const float *grid, *up;
+    __device__ __forceinline__ void add(int64_t pos, int64_t cell, float acc[7]) const {
+        const float4 p = *reinterpret_cast<const float4 *>(pay + pos);
+        quadric_normal_grad(load_quadric(grid, cell), f3{p.x, p.y, p.z},
+                            load3(up, __float_as_uint(p.w)), acc);
     }
 };
 
@@ -217,12 +242,12 @@ struct NormalSamples {
 // wave where such a run starts adds its tail sum and the head sums of the
 // waves the run covers, in order, to the cell.  The same input therefore
 // always gives the same bits.
-// Keys are cells, or carry bits above the cell that `cmask` clears (the
-// (parity class, cell) keys of GridInterpolationBackward's sort).
-template <class Src>
+// `keys` maps a key to its cell (CellKeys, or GridInterpolationBackward's
+// ClassKeys when a run sum reuses that sort).
+template <class Src, class Keys>
 __global__ void __launch_bounds__(256) quadric_seg_kernel(const uint32_t *__restrict__ key,
                                                           Src src, int64_t n, uint32_t invalid,
-                                                          uint32_t cmask,
+                                                          Keys keys,
                                                           float *__restrict__ gq,
                                                           float *__restrict__ head,
                                                           float *__restrict__ tail) {
@@ -233,7 +258,7 @@ __global__ void __launch_bounds__(256) quadric_seg_kernel(const uint32_t *__rest
     const int64_t p = p0 + lane;
     const uint32_t k = p < n ? key[p] : invalid;
     float v[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (k != invalid) src.add(p, k & cmask, v);
+    if (k != invalid) src.add(p, keys.cell(k), v);
     const uint32_t kp = __shfl_up(k, 1, kWave), kn = __shfl_down(k, 1, kWave);
     const bool start = lane == 0 || kp != k;
     const bool last = lane == kWave - 1 || kn != k;
@@ -265,7 +290,7 @@ __global__ void __launch_bounds__(256) quadric_seg_kernel(const uint32_t *__rest
 #pragma unroll
             for (int j = 0; j < 7; ++j) dst[j] = v[j];
         } else {
-            float *g = gq + 7 * (int64_t)(k & cmask);
+            float *g = gq + 7 * keys.cell(k);
 #pragma unroll
             for (int j = 0; j < 7; ++j) g[j] += v[j];
         }
@@ -273,9 +298,10 @@ __global__ void __launch_bounds__(256) quadric_seg_kernel(const uint32_t *__rest
 }
 
 // Pass 2: one thread per wave of pass 1.
+template <class Keys>
 __global__ void __launch_bounds__(256) quadric_seg_join_kernel(const uint32_t *__restrict__ key,
                                                                int64_t n, uint32_t invalid,
-                                                               uint32_t cmask,
+                                                               Keys keys,
                                                                const float *__restrict__ head,
                                                                const float *__restrict__ tail,
                                                                float *__restrict__ gq) {
@@ -294,34 +320,34 @@ __global__ void __launch_bounds__(256) quadric_seg_join_kernel(const uint32_t *_
         const int64_t q = (u + 1) * kWave;
         if (q >= n || key[q] != k) break;
     }
-    float *g = gq + 7 * (int64_t)(k & cmask);
+    float *g = gq + 7 * keys.cell(k);
 #pragma unroll
     for (int j = 0; j < 7; ++j) g[j] += t[j];
 }
 
 // Both passes over n sorted records; head and tail hold 7 floats per wave.
-template <class Src>
+template <class Src, class Keys = CellKeys>
 int quadric_seg_sum(const uint32_t *key, Src src, int64_t n, uint32_t invalid, float *gq,
-                    float *head, float *tail, hipStream_t s, uint32_t cmask = 0xFFFFFFFFu) {
+                    float *head, float *tail, hipStream_t s, Keys keys = Keys{}) {
     if (n == 0) return QSDVR_OK;
-    quadric_seg_kernel<<<grid_1d(n, 256), 256, 0, s>>>(key, src, n, invalid, cmask, gq, head,
+    quadric_seg_kernel<<<grid_1d(n, 256), 256, 0, s>>>(key, src, n, invalid, keys, gq, head,
                                                        tail);
     QSDVR_LAUNCHED();
     const int64_t waves = (n + kWave - 1) / kWave;
-    quadric_seg_join_kernel<<<grid_1d(waves, 256), 256, 0, s>>>(key, n, invalid, cmask, head,
+    quadric_seg_join_kernel<<<grid_1d(waves, 256), 256, 0, s>>>(key, n, invalid, keys, head,
                                                                 tail, gq);
     QSDVR_LAUNCHED();
     return QSDVR_OK;
 }
 
 // Class boundaries in the sorted appearance keys: bounds[c] = first
-// position whose key is >= c << cbits (c = 0..8; bounds[8] starts the
+// position whose key is >= c * stride (c = 0..8; bounds[8] starts the
 // invalid keys).
-__global__ void det_class_bounds_kernel(const uint32_t *__restrict__ key, int64_t n, int cbits,
-                                        uint32_t *__restrict__ bounds) {
+__global__ void det_class_bounds_kernel(const uint32_t *__restrict__ key, int64_t n,
+                                        uint32_t stride, uint32_t *__restrict__ bounds) {
     const int c = threadIdx.x;
     if (c > 8) return;
-    const uint32_t target = (uint32_t)c << cbits;
+    const uint32_t target = (uint32_t)c * stride;
     int64_t lo = 0, hi = n;
     while (lo < hi) {
         const int64_t mid = (lo + hi) / 2;
@@ -354,15 +380,6 @@ struct AppRecords {
     // multiples of 4 floats)
     __device__ __forceinline__ float4 grad4(uint32_t r, int q) const {
         return *reinterpret_cast<const float4 *>(aval + (int64_t)r * det_app_stride(C) + 4 + 4 * q);
-    }
-};
-
-template <int C>
-struct InterpSamples {
-    const float *pts, *up;
-    __device__ __forceinline__ f3 local(uint32_t r) const { return load3(pts, r); }
-    __device__ __forceinline__ float grad(uint32_t r, int ch) const {
-        return up[(int64_t)r * C + ch];
     }
 };
 
@@ -501,20 +518,19 @@ __global__ void __launch_bounds__(256) det_app_reduce32x4_kernel(
     }
 }
 
-// det_app_reduce_kernel for the op surface's InterpSamples with the records
-// of a run fetched eight at a time: the group's first 8 lanes load the next
-// 8 sample indices, every lane then issues the 8 upstream values and points
-// together, so a run costs two memory round trips per 8 records instead of
-// one dependent chain per record.  Each (vertex, channel) sum still runs over
-// the run's records in order, so the result is bitwise that of
-// det_app_reduce_kernel.  The walk loop is uniform across the wavefront
-// (groups whose run is done keep issuing masked loads) so that the index
-// broadcast sees every lane.
+// The op surface's appearance runs (GridInterpolationBackward): as
+// det_app_reduce_kernel, with the run's records fetched eight at a time --
+// every lane of the group loads the next 8 (point, sample index) records,
+// which the sort left in order, and then issues the 8 upstream rows they
+// index together, so a run costs two memory round trips per 8 records
+// instead of one dependent chain per record.  Each (vertex, channel) sum
+// runs over the run's records in order.  The walk loop is uniform across
+// the wavefront (groups whose run is done issue masked loads).
 template <int C>
 __global__ void __launch_bounds__(256) op_app_reduce_kernel(
-    const uint32_t *__restrict__ key, const uint32_t *__restrict__ idx,
-    const float *__restrict__ pts, const float *__restrict__ up,
-    const uint32_t *__restrict__ bounds, int cls, int cbits, int R, float *__restrict__ gdata) {
+    const uint32_t *__restrict__ key, const PointRec *__restrict__ pay,
+    const float *__restrict__ up, const uint32_t *__restrict__ bounds, int cls, ClassKeys ck,
+    float *__restrict__ gdata) {
     static_assert(C <= kWave, "one lane per channel");
     constexpr int NG = C <= kWave / 2 ? 2 : 1;  // runs in flight per wave
     constexpr int GW = kWave / NG;
@@ -522,8 +538,7 @@ __global__ void __launch_bounds__(256) op_app_reduce_kernel(
     const int64_t lo = bounds[cls], hi = bounds[cls + 1];
     const int lane = threadIdx.x % kWave;
     const int grp = lane / GW, ch = lane % GW;
-    const uint32_t cmask = (1u << cbits) - 1u;
-    const int64_t RR = (int64_t)R * R;
+    const int64_t R = ck.R, RR = (int64_t)R * R;
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     for (int64_t p0 = lo + wave * kWave; p0 < hi; p0 += nwaves * kWave) {
@@ -554,34 +569,39 @@ __global__ void __launch_bounds__(256) op_app_reduce_kernel(
             float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             for (int64_t i = start;; i += B) {
                 if (!__any(i < end)) break;
-                // lane ch < B of the group fetches the index of record i + ch
-                const uint32_t rr = (ch < B && i + ch < end) ? idx[i + ch] : 0u;
-                uint32_t r[B];
+                // unconditional loads (a finished lane reads the class's
+                // first record, a lane past the channels the row's last),
+                // so that all of them issue before the first wait
+                float4 rec[B];
 #pragma unroll
-                for (int b = 0; b < B; ++b) r[b] = __shfl(rr, grp * GW + b, kWave);
+                for (int b = 0; b < B; ++b)
+                    rec[b] = *reinterpret_cast<const float4 *>(pay + (i + b < end ? i + b : lo));
+                __builtin_amdgcn_sched_barrier(0);  // keep the 8 record loads together
+                const int cc = ch < C ? ch : C - 1;
                 float g[B];
-                f3 pt[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b)
+                    g[b] = up[(int64_t)__float_as_uint(rec[b].w) * C + cc];
+                __builtin_amdgcn_sched_barrier(0);  // and the 8 row loads
 #pragma unroll
                 for (int b = 0; b < B; ++b) {
-                    const bool ok = i + b < end;
-                    g[b] = ok && ch < C ? up[(int64_t)r[b] * C + ch] : 0.0f;
-                    pt[b] = ok ? load3(pts, r[b]) : f3{0.0f, 0.0f, 0.0f};
-                }
+                    // a select, not a branch (which would let the compiler
+                    // sink the record's loads into it)
+                    const bool ok = i + b < end && ch < C;
+                    float w[8];
+                    trilinear_weights(f3{rec[b].x, rec[b].y, rec[b].z}, w);
 #pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    if (i + b < end) {
-                        float w[8];
-                        trilinear_weights(pt[b], w);
-#pragma unroll
-                        for (int v = 0; v < 8; ++v) acc[v] += w[v] * g[b];
+                    for (int v = 0; v < 8; ++v) {
+                        const float t = acc[v] + w[v] * g[b];
+                        acc[v] = ok ? t : acc[v];
                     }
                 }
             }
             if (my >= 0 && ch < C) {
-                const int64_t cell = hk & cmask;
+                const int64_t cell = ck.cell(hk);
 #pragma unroll
                 for (int v = 0; v < 8; ++v) {
-                    const int64_t row = cell + (v & 1) + ((v >> 1) & 1) * (int64_t)R + (v >> 2) * RR;
+                    const int64_t row = cell + (v & 1) + ((v >> 1) & 1) * R + (v >> 2) * RR;
                     gdata[row * C + ch] += acc[v];
                 }
             }
@@ -604,7 +624,7 @@ int det_reduce(DetBuffers &b, int64_t T, float *gq, float *gdata, hipStream_t s)
     int st = quadric_seg_sum(b.qkey_s, QuadricRecords{b.qval, b.qidx_s}, 2 * T, (uint32_t)R3, gq,
                              b.qhead, b.qtail, s);
     if (st != QSDVR_OK) return st;
-    det_class_bounds_kernel<<<1, 64, 0, s>>>(b.akey_s, T, b.cbits, b.bounds);
+    det_class_bounds_kernel<<<1, 64, 0, s>>>(b.akey_s, T, 1u << b.cbits, b.bounds);
     QSDVR_LAUNCHED();
     // each class holds about T/8 records; the waves stride over its range
     const int64_t chunks = (T / 8 + kWave - 1) / kWave;
@@ -629,26 +649,27 @@ int det_reduce(DetBuffers &b, int64_t T, float *gq, float *gdata, hipStream_t s)
 
 // ------------------------------------- op-surface sorted scatters ----
 // GridInterpolationBackward, SampleSDFBackward and SampleNormalBackward as
-// sorted-segment sums.  GridInterpolationBackward and SampleNormalBackward
-// sort sample indices by key and gather the records in sorted order;
-// SampleSDFBackward sorts its 16-B records (point, upstream) with the keys.
+// sorted-segment sums.  GridInterpolationBackward sorts (point, sample
+// index) records by ClassKeys and SampleSDFBackward its 16-B records (point,
+// upstream) by cell, so that their run sums read the points in order;
+// SampleNormalBackward sorts sample indices by cell and gathers its records
+// in sorted order, or sums its runs on the sort of a GridInterpolationBackward
+// over the same samples.
 
-// key of sample i: (parity class << cbits) | cell, invalid 8 << cbits
+// ClassKeys key and (point, index) record of sample i
 __global__ void __launch_bounds__(256) op_interp_keys_kernel(const int32_t *__restrict__ idx,
-                                                             int64_t n, int R, int cbits,
+                                                             const float *__restrict__ pts,
+                                                             int64_t n, ClassKeys ck,
                                                              uint32_t *__restrict__ key,
-                                                             uint32_t *__restrict__ val) {
+                                                             PointRec *__restrict__ pay) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int64_t cell = idx[i];
-    uint32_t k = 8u << cbits;
-    if (cell >= 0 && cell < (int64_t)R * R * R) {
-        const int64_t x = cell % R, y = (cell / R) % R, z = cell / ((int64_t)R * R);
-        const uint32_t cls = (uint32_t)((x & 1) | ((y & 1) << 1) | ((z & 1) << 2));
-        k = (cls << cbits) | (uint32_t)cell;
-    }
+    const int64_t cell = idx[i], R = ck.R;
+    uint32_t k = 8u * ck.h3;
+    if (cell >= 0 && cell < R * R * R) k = ck.key(cell % R, (cell / R) % R, cell / (R * R));
     key[i] = k;
-    val[i] = (uint32_t)i;
+    const f3 p = load3(pts, i);
+    *reinterpret_cast<float4 *>(pay + i) = float4{p.x, p.y, p.z, __uint_as_float((uint32_t)i)};
 }
 
 // key of sample i: its cell, invalid `cells`; the record (point and
@@ -700,17 +721,10 @@ size_t sort_temp_bytes_v(int64_t n, int end_bit) {
     return t;
 }
 
-// smallest b with 2^b >= n (cell bits of the interpolation keys)
-int bits_at_least(uint64_t n) {
-    int b = 1;
-    while ((1ull << b) < n) ++b;
-    return b;
-}
-
 size_t op_temp_bytes(int64_t n, int64_t cells) {
     const int qb = bits_above((uint64_t)cells);
-    return std::max({sort_temp_bytes(n, bits_at_least((uint64_t)cells) + 4),
-                     sort_temp_bytes(n, qb), sort_temp_bytes_v<float4>(n, qb)});
+    return std::max({sort_temp_bytes(n, qb), sort_temp_bytes_v<float4>(n, qb),
+                     sort_temp_bytes_v<PointRec>(n, qb)});
 }
 
 size_t op_carve(void *ws, size_t bytes, int64_t n, int64_t cells, OpBuffers &b) {
@@ -770,23 +784,19 @@ int qsdvr_grid_interpolation_backward_sorted(float *grad_grid, const float *poin
                   "qsdvr_grid_interpolation_backward_sorted: null pointer");
     const int R = vertex_reso - 1;
     const int64_t cells = (int64_t)R * R * R;
-    // key = (parity class << cbits) | cell, with class 8 marking an invalid
-    // sample: cbits + 4 bits must fit the 32-bit key
-    QSDVR_REQUIRE(bits_at_least((uint64_t)cells) + 4 <= 32,
-                  "qsdvr_grid_interpolation_backward_sorted: reso too large for 32-bit keys");
     OpBuffers b;
     int st = op_prepare(workspace, workspace_bytes, n, cells,
                         b, "qsdvr_grid_interpolation_backward_sorted");
     if (st != QSDVR_OK) return st;
     hipStream_t s = as_stream(stream);
-    const int cbits = bits_at_least((uint64_t)cells);
-    uint32_t *val = reinterpret_cast<uint32_t *>(b.val), *val_s = reinterpret_cast<uint32_t *>(b.val_s);
-    op_interp_keys_kernel<<<grid_1d(n, 256), 256, 0, s>>>(index, n, R, cbits, b.key, val);
+    const ClassKeys ck = class_keys(R);
+    PointRec *pay = reinterpret_cast<PointRec *>(b.val), *pay_s = reinterpret_cast<PointRec *>(b.val_s);
+    op_interp_keys_kernel<<<grid_1d(n, 256), 256, 0, s>>>(index, point, n, ck, b.key, pay);
     QSDVR_LAUNCHED();
     size_t tb = b.temp_bytes;
-    QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.key, b.key_s, val, val_s, (size_t)n, 0u,
-                                        (unsigned)(cbits + 4), s));
-    det_class_bounds_kernel<<<1, 64, 0, s>>>(b.key_s, n, cbits, b.bounds);
+    QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.key, b.key_s, pay, pay_s, (size_t)n, 0u,
+                                        (unsigned)bits_above((uint64_t)cells), s));
+    det_class_bounds_kernel<<<1, 64, 0, s>>>(b.key_s, n, ck.h3, b.bounds);
     QSDVR_LAUNCHED();
     const int64_t chunks = (n / 8 + kWave - 1) / kWave;
     const unsigned blocks =
@@ -795,11 +805,11 @@ int qsdvr_grid_interpolation_backward_sorted(float *grad_grid, const float *poin
         // (one lane per channel here: a frame's samples sorted by cell make
         // long, uneven runs, and a wavefront of 8 runs waits for its longest)
         if (channels == 32)
-            op_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(b.key_s, val_s, point, grad_out,
-                                                           b.bounds, cls, cbits, R, grad_grid);
+            op_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(b.key_s, pay_s, grad_out, b.bounds,
+                                                           cls, ck, grad_grid);
         else
-            op_app_reduce_kernel<53><<<blocks, 256, 0, s>>>(b.key_s, val_s, point, grad_out,
-                                                           b.bounds, cls, cbits, R, grad_grid);
+            op_app_reduce_kernel<53><<<blocks, 256, 0, s>>>(b.key_s, pay_s, grad_out, b.bounds,
+                                                           cls, ck, grad_grid);
         QSDVR_LAUNCHED();
     }
     return QSDVR_OK;
@@ -855,26 +865,23 @@ int qsdvr_sample_normal_backward_sorted(const float *grad_out, const float *sdf_
 }
 
 int qsdvr_sample_normal_backward_presorted(const float *grad_out, const float *sdf_grid,
-                                           const float *point, const int32_t *index,
                                            float *grad_grid, int64_t n, int32_t reso,
                                            void *workspace, size_t workspace_bytes,
                                            qsdvr_stream stream) {
     QSDVR_REQUIRE(n >= 0 && reso >= 2 && reso % 2 == 0,
                   "qsdvr_sample_normal_backward_presorted: bad sizes (even reso)");
     if (n == 0) return QSDVR_OK;
-    QSDVR_REQUIRE(grad_out && sdf_grid && point && index && grad_grid,
+    QSDVR_REQUIRE(grad_out && sdf_grid && grad_grid,
                   "qsdvr_sample_normal_backward_presorted: null pointer");
     const int64_t cells = (int64_t)reso * reso * reso;
-    const int cbits = bits_at_least((uint64_t)cells);
-    QSDVR_REQUIRE(cbits + 4 <= 32, "qsdvr_sample_normal_backward_presorted: reso too large");
     OpBuffers b;
     int st = op_prepare(workspace, workspace_bytes, n, cells, b,
                         "qsdvr_sample_normal_backward_presorted");
     if (st != QSDVR_OK) return st;
-    const uint32_t *val_s = reinterpret_cast<const uint32_t *>(b.val_s);
-    return quadric_seg_sum(b.key_s, NormalSamples{val_s, sdf_grid, point, grad_out}, n,
-                           8u << cbits, grad_grid, b.head, b.tail, as_stream(stream),
-                           (1u << cbits) - 1u);
+    const PointRec *pay_s = reinterpret_cast<const PointRec *>(b.val_s);
+    return quadric_seg_sum(b.key_s, NormalPresorted{pay_s, sdf_grid, grad_out}, n,
+                           (uint32_t)cells, grad_grid, b.head, b.tail, as_stream(stream),
+                           class_keys(reso));
 }
 
 }  // extern "C"

@@ -183,7 +183,7 @@ def GridInterpolationBackward(dataGrid, pointList, indexList, out, reso):
         ws = _workspace(ws_n)
         _lib.call("qsdvr_grid_interpolation_backward_sorted", _ptr(dataGrid), _ptr(pointList),
                   _ptr(indexList), _ptr(out), n, R1, Cc, _ptr(ws), ws_n, _stream())
-        _remember_render_sort(indexList, n, R1 - 1, ws, ws_n)
+        _remember_render_sort(pointList, indexList, n, R1 - 1, ws, ws_n)
         return
     _lib.call("qsdvr_grid_interpolation_backward", _ptr(dataGrid), _ptr(pointList),
               _ptr(indexList), _ptr(out), n, R1, Cc, _stream())
@@ -319,12 +319,11 @@ def SampleNormalBackward(out, sdfGrid, pointList, indexList, sdfGradGrid):
     if sdfGradGrid.numel() != sdfGrid.numel():
         raise ValueError("sdfGradGrid must match sdfGrid")
     if quadric_backward_path(n, sdfGrid) == "sorted":
-        reuse = _take_render_sort(indexList, n, cells)
-        if reuse is not None:  # GridInterpolationBackward already sorted these indices
+        reuse = _take_render_sort(pointList, indexList, n, cells)
+        if reuse is not None:  # GridInterpolationBackward already sorted these samples
             R, ws, ws_n = reuse
             _lib.call("qsdvr_sample_normal_backward_presorted", _ptr(out), _ptr(sdfGrid),
-                      _ptr(pointList), _ptr(indexList), _ptr(sdfGradGrid), n, R, _ptr(ws),
-                      ws_n, _stream())
+                      _ptr(sdfGradGrid), n, R, _ptr(ws), ws_n, _stream())
             return
         ws_n = _sorted_workspace(n, cells)
         ws = _workspace(ws_n)
@@ -364,13 +363,6 @@ def _workspace(nbytes):
 SCATTER = "sorted"
 
 
-def _bits_at_least(v):
-    b = 1
-    while (1 << b) < v:
-        b += 1
-    return b
-
-
 def _sorted_fits(n, cells):
     """The sorted-segment kernels key samples by cell in 32 bits and count
     them in 32 bits."""
@@ -379,14 +371,12 @@ def _sorted_fits(n, cells):
 
 def interp_backward_path(n, vertex_reso, channels):
     """"sorted" or "atomic": which GridInterpolationBackward kernel runs for
-    ``SCATTER``.  The sorted form keys a sample by (cell parity class, cell)
-    in 32 bits (3 class bits + an invalid flag above the cell bits) and
-    needs an even cell resolution (cells of one parity class then share no
-    vertex row) and 32 or 53 channels."""
+    ``SCATTER``.  The sorted form orders the samples by cell parity class
+    and cell (keys up to R^3) and needs an even cell resolution (cells of
+    one parity class then share no vertex row) and 32 or 53 channels."""
     R = int(vertex_reso) - 1
-    cells = R ** 3
-    ok = (SCATTER == "sorted" and _sorted_fits(n, cells) and channels in (32, 53) and R >= 2
-          and R % 2 == 0 and _bits_at_least(cells) + 4 <= 32)
+    ok = (SCATTER == "sorted" and _sorted_fits(n, R ** 3) and channels in (32, 53) and R >= 2
+          and R % 2 == 0)
     return "sorted" if ok else "atomic"
 
 
@@ -398,36 +388,39 @@ def quadric_backward_path(n, sdfGrid):
 
 
 # The reference's backward (models/render.py) runs GridInterpolationBackward
-# and then SampleNormalBackward on the same render index list.  The sorted
-# GridInterpolationBackward orders those indices by (parity class, cell),
-# which also groups equal cells, so SampleNormalBackward can sum its runs on
-# that order instead of sorting again.  The workspace holding it is kept
-# here (one entry) with a weak reference to the index tensor, its version
-# counter (an in-place change invalidates it), its size and resolution, the
-# stream and the op-call count: only the very next op call, if it is
-# SampleNormalBackward on that tensor, takes it.  (The reused order sums a
+# and then SampleNormalBackward on the same render point and index lists.
+# The sorted GridInterpolationBackward orders those samples (point and index)
+# by (parity class, cell), which also groups equal cells, so
+# SampleNormalBackward can sum its runs on that order instead of sorting
+# again.  The workspace holding it is kept here (one entry) with weak
+# references to the point and index tensors, their version counters (an
+# in-place change invalidates it), the size and resolution, the stream and
+# the op-call count: only the very next op call, if it is
+# SampleNormalBackward on those tensors, takes it.  (The reused order sums a
 # cell's samples in a different tree than SampleNormalBackward's own sort,
 # so the bits then depend on the preceding call -- the same sequence of
 # calls still gives the same bits.)
 _render_sort = None
 
 
-def _remember_render_sort(index, n, R, ws, ws_n):
+def _remember_render_sort(points, index, n, R, ws, ws_n):
     global _render_sort
     import weakref
-    _render_sort = (weakref.ref(index), index._version, int(n), int(R), ws, int(ws_n),
-                    torch.cuda.current_stream().cuda_stream, _calls)
+    _render_sort = (weakref.ref(points), points._version, weakref.ref(index), index._version,
+                    int(n), int(R), ws, int(ws_n), torch.cuda.current_stream().cuda_stream,
+                    _calls)
 
 
-def _take_render_sort(index, n, cells):
+def _take_render_sort(points, index, n, cells):
     """(R, workspace, bytes) of a matching GridInterpolationBackward sort, or
     None; the entry is consumed."""
     global _render_sort
     ent, _render_sort = _render_sort, None
     if ent is None:
         return None
-    ref, version, n0, R, ws, ws_n, stream, call = ent
-    if (call + 1 != _calls or ref() is not index or index._version != version or n0 != int(n)
+    pref, pversion, iref, iversion, n0, R, ws, ws_n, stream, call = ent
+    if (call + 1 != _calls or pref() is not points or points._version != pversion
+            or iref() is not index or index._version != iversion or n0 != int(n)
             or R ** 3 != int(cells) or stream != torch.cuda.current_stream().cuda_stream):
         return None
     return R, ws, ws_n

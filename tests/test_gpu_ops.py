@@ -278,7 +278,8 @@ def test_normal_backward_reuses_the_interpolation_sort(monkeypatch):
     GridInterpolationBackward on the same render indices sums its runs on
     that call's (parity class, cell) order instead of sorting again; it
     equals SampleNormalBackward's own sorted sums to 1e-4, repeats bitwise,
-    and is not used when another call intervenes or the indices changed."""
+    and is not used when another call intervenes, the indices changed or the
+    points are another tensor."""
     from qsdvr_amd import _lib
     from qsdvr_amd.camera import Camera
     w = scenes.WORKLOADS["plain"]
@@ -324,4 +325,9 @@ def test_normal_backward_reuses_the_interpolation_sort(monkeypatch):
     interp()
     ri.add_(0)                                                        # in-place: new version
     assert torch.equal(normal(), own)
+    assert "qsdvr_sample_normal_backward_presorted" not in names
+    interp()
+    gn = torch.zeros_like(grid)
+    qsdvr.SampleNormalBackward(up_n, grid, rp.clone(), ri, gn)        # other point tensor
+    assert torch.equal(gn, own)
     assert "qsdvr_sample_normal_backward_presorted" not in names

@@ -52,16 +52,17 @@ def test_camera_args_mirror():
 
 
 def test_interpolation_backward_path_choice(monkeypatch):
-    """GridInterpolationBackward takes the sorted scatter wherever its 32-bit
-    (parity class, cell) keys fit at an even resolution, and the atomic
-    kernel (no size limit) everywhere else -- never an error at sizes the
-    reference runs."""
+    """GridInterpolationBackward takes the sorted scatter wherever its keys
+    (parity class and cell, up to R^3 < 2^31) fit at an even resolution, and
+    the atomic kernel (no size limit) everywhere else -- never an error at
+    sizes the reference runs."""
     n = 1000
     assert qsdvr.SCATTER == "sorted"
     assert qsdvr.interp_backward_path(n, 129, 32) == "sorted"
     assert qsdvr.interp_backward_path(n, 385, 53) == "sorted"
-    assert qsdvr.interp_backward_path(n, 513, 32) == "sorted"    # 2^27 cells: 31-bit keys
-    assert qsdvr.interp_backward_path(n, 1025, 32) == "atomic"   # 2^30 cells: 34-bit keys
+    assert qsdvr.interp_backward_path(n, 513, 32) == "sorted"    # 2^27 cells
+    assert qsdvr.interp_backward_path(n, 1291, 32) == "sorted"   # 1290^3 < 2^31 - 1 cells
+    assert qsdvr.interp_backward_path(n, 1293, 32) == "atomic"   # 1292^3 > 2^31 - 1 cells
     assert qsdvr.interp_backward_path(n, 128, 32) == "atomic"    # odd cell resolution
     assert qsdvr.interp_backward_path(n, 129, 40) == "atomic"    # channel count
     assert qsdvr.interp_backward_path(0, 129, 32) == "atomic"
