@@ -609,6 +609,130 @@ __global__ void __launch_bounds__(256) op_app_reduce_kernel(
     }
 }
 
+// op_app_reduce_kernel for C = 32 with one run per wavefront and lane =
+// (channel quad q, record slot b): each step takes 8 records of the run --
+// one 16-B load brings their (point, index) records, one more their full
+// 128-B upstream rows -- and lane (q, b) adds record b's 8 vertex weights
+// times its 4 channels to 32 running sums (records b, b + 8, ... of the run,
+// two sets of 8 per round trip).
+// A memory instruction costs the address unit the same however few lanes
+// or addresses it carries, so this needs 2 per 8 records where a lane per
+// channel needs 2 per record.  At the run's end a reduce-scatter over the 8
+// slots (three DPP exchange-and-add steps) leaves vertex b's 4 channels in
+// lane (q, b), and one read-modify-write per lane adds the 8 vertex rows.
+// Runs of a frame are long (a median of about 33 samples per cell at
+// 128^3, 800x800), so few slots idle.  A fixed order: bitwise reproducible.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
+__global__ void __launch_bounds__(256) op_app_reduce32_kernel(
+    const uint32_t *__restrict__ key, const PointRec *__restrict__ pay,
+    const float4 *__restrict__ up, const uint32_t *__restrict__ bounds, int cls, ClassKeys ck,
+    float *__restrict__ gdata) {
+    constexpr int kHalfMirror = 0x141, kSwap2 = 0x4E, kSwap1 = 0xB1;  // DPP controls
+    const int64_t lo = bounds[cls], hi = bounds[cls + 1];
+    const int lane = threadIdx.x % kWave;
+    const int q = lane >> 3, b = lane & 7;
+    const int64_t R = ck.R, RR = (int64_t)R * R;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    for (int64_t p0 = lo + wave * kWave; p0 < hi; p0 += nwaves * kWave) {
+        const int64_t p = p0 + lane;
+        const uint32_t k = p < hi ? key[p] : 0xFFFFFFFFu;
+        const bool boundary = p < hi && (p == lo || key[p - 1] != k);
+        const unsigned long long bmask = __ballot(boundary);
+        if (bmask == 0) continue;  // inside a run that started in an earlier chunk
+        const int last = 63 - __clzll((long long)bmask);
+        const int64_t tail_end = run_end(key, p0 + kWave, hi, __shfl(k, last, kWave));
+        unsigned long long heads = bmask;
+        while (heads != 0) {
+            const int h = __ffsll((long long)heads) - 1;
+            heads &= heads - 1;
+            const int64_t start = p0 + h;
+            const int64_t end = heads != 0 ? p0 + (__ffsll((long long)heads) - 1) : tail_end;
+            const uint32_t hk = __shfl(k, h, kWave);
+            // lane (q, b)'s quad of vertex row b, read now: no other run of
+            // this class touches the cell's rows, and the read is then done
+            // by the time the sums are
+            const int64_t row = ck.cell(hk) + (b & 1) + ((b >> 1) & 1) * R + (b >> 2) * RR;
+            float4 *dst = reinterpret_cast<float4 *>(gdata + row * 32) + q;
+            float4 o = *dst;
+            float acc[8][4];
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[v][c] = 0.0f;
+            // 16 records a step (slot b takes i0 + b and i0 + 8 + b), the next
+            // step's records loaded while this step's rows are; a slot past the
+            // run's end reads the run's first record and adds nothing (its row
+            // is zeroed)
+            auto rec_at = [&](int64_t i) {
+                return *reinterpret_cast<const float4 *>(pay + (i < end ? i : start));
+            };
+            float4 na = rec_at(start + b), nb = rec_at(start + 8 + b);
+            for (int64_t i0 = start; i0 < end; i0 += 16) {
+                const float4 ra = na, rb = nb;
+                float4 ga = up[(int64_t)__float_as_uint(ra.w) * 8 + q];
+                float4 gb = up[(int64_t)__float_as_uint(rb.w) * 8 + q];
+                na = rec_at(i0 + 16 + b);
+                nb = rec_at(i0 + 24 + b);
+                if (i0 + b >= end) ga = float4{0.0f, 0.0f, 0.0f, 0.0f};
+                if (i0 + 8 + b >= end) gb = float4{0.0f, 0.0f, 0.0f, 0.0f};
+                float wa[8], wb[8];
+                trilinear_weights(f3{ra.x, ra.y, ra.z}, wa);
+                trilinear_weights(f3{rb.x, rb.y, rb.z}, wb);
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    acc[v][0] += wa[v] * ga.x;
+                    acc[v][1] += wa[v] * ga.y;
+                    acc[v][2] += wa[v] * ga.z;
+                    acc[v][3] += wa[v] * ga.w;
+                }
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    acc[v][0] += wb[v] * gb.x;
+                    acc[v][1] += wb[v] * gb.y;
+                    acc[v][2] += wb[v] * gb.z;
+                    acc[v][3] += wb[v] * gb.w;
+                }
+            }
+            // reduce-scatter over the 8 slots: split the vertices by bit 2
+            // with slot 7 - b, by bit 1 with slot b ^ 2, by bit 0 with b ^ 1
+            const bool h2 = (b & 4) != 0, h1 = (b & 2) != 0, h0 = (b & 1) != 0;
+            float k1[4][4], k2[2][4], tot[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float keep = h2 ? acc[4 + j][c] : acc[j][c];
+                    const float send = h2 ? acc[j][c] : acc[4 + j][c];
+                    k1[j][c] = keep + dpp_f<kHalfMirror>(send);
+                }
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float keep = h1 ? k1[2 + j][c] : k1[j][c];
+                    const float send = h1 ? k1[j][c] : k1[2 + j][c];
+                    k2[j][c] = keep + dpp_f<kSwap2>(send);
+                }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float keep = h0 ? k2[1][c] : k2[0][c];
+                const float send = h0 ? k2[0][c] : k2[1][c];
+                tot[c] = keep + dpp_f<kSwap1>(send);
+            }
+            o.x += tot[0];
+            o.y += tot[1];
+            o.z += tot[2];
+            o.w += tot[3];
+            *dst = o;
+        }
+    }
+}
+
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 int det_reduce(DetBuffers &b, int64_t T, float *gq, float *gdata, hipStream_t s) {
@@ -801,10 +925,16 @@ int qsdvr_grid_interpolation_backward_sorted(float *grad_grid, const float *poin
     const int64_t chunks = (n / 8 + kWave - 1) / kWave;
     const unsigned blocks =
         (unsigned)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 4096));
+    const bool quads = channels == 32 && aligned16(grad_out) && aligned16(grad_grid);
     for (int cls = 0; cls < 8; ++cls) {
-        // (one lane per channel here: a frame's samples sorted by cell make
-        // long, uneven runs, and a wavefront of 8 runs waits for its longest)
-        if (channels == 32)
+        // (one run per wavefront or per half: a frame's samples sorted by cell
+        // make long, uneven runs, and a wavefront of 8 runs waits for its
+        // longest)
+        if (quads)
+            op_app_reduce32_kernel<<<blocks, 256, 0, s>>>(
+                b.key_s, pay_s, reinterpret_cast<const float4 *>(grad_out), b.bounds, cls, ck,
+                grad_grid);
+        else if (channels == 32)
             op_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(b.key_s, pay_s, grad_out, b.bounds,
                                                            cls, ck, grad_grid);
         else

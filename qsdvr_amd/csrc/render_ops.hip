@@ -41,9 +41,11 @@ __global__ void __launch_bounds__(256) interp_fwd_kernel(const float *__restrict
 }
 
 // C = 32: lane = (sample slot, channel quad), 8 lanes per sample with one
-// 16-B load per vertex row, 8 samples per wave step and ITER steps per wave
-// (the loads of all steps in flight together); the same lerp order per
-// channel as interp_fwd_kernel.
+// 16-B load per vertex row, 8 samples per wave step and ITER steps per wave;
+// the same lerp order per channel as interp_fwd_kernel.  The loads carry no
+// branches (a sample past the end reads the last sample, an invalid cell
+// reads cell 0 and stores NaN), so that a lane issues the indices of all its
+// steps, then the points and rows of all its steps, before it waits.
 __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float w) {
     return float4{lerp_ref(a.x, b.x, w), lerp_ref(a.y, b.y, w), lerp_ref(a.z, b.z, w),
                   lerp_ref(a.w, b.w, w)};
@@ -58,26 +60,39 @@ __global__ void __launch_bounds__(256) interp_fwd32_kernel(const float4 *__restr
     const int lane = threadIdx.x % kWave, q = lane & 7, slot = lane >> 3;
     const int64_t wave = (xcd_block() * blockDim.x + threadIdx.x) / kWave;
     const int64_t cells = (int64_t)R * R * R, RR = (int64_t)R * R;
+    int64_t s[ITER], cell[ITER];
 #pragma unroll
     for (int it = 0; it < ITER; ++it) {
-        const int64_t s = (wave * ITER + it) * 8 + slot;
-        if (s >= n) continue;
-        const int64_t cell = idx[s];
-        if (cell < 0 || cell >= cells) {
+        const int64_t si = (wave * ITER + it) * 8 + slot;
+        s[it] = si < n ? si : n - 1;
+        cell[it] = idx[s[it]];
+    }
+    f3 p[ITER];
+    float4 v[ITER][8];
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+        p[it] = load3(pts, s[it]);
+        const bool ok = cell[it] >= 0 && cell[it] < cells;
+        const float4 *b = data + (ok ? cell[it] : 0) * 8 + q;
+        const int64_t off[8] = {0, 1, R, R + 1, RR, RR + 1, RR + R, RR + R + 1};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[it][k] = b[off[k] * 8];
+    }
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+        const int64_t si = (wave * ITER + it) * 8 + slot;
+        const float4 a00 = lerp4(v[it][0], v[it][1], p[it].x);
+        const float4 a01 = lerp4(v[it][2], v[it][3], p[it].x);
+        const float4 a0 = lerp4(a00, a01, p[it].y);
+        const float4 a10 = lerp4(v[it][4], v[it][5], p[it].x);
+        const float4 a11 = lerp4(v[it][6], v[it][7], p[it].x);
+        const float4 a1 = lerp4(a10, a11, p[it].y);
+        float4 r = lerp4(a0, a1, p[it].z);
+        if (!(cell[it] >= 0 && cell[it] < cells)) {
             const float nan = __builtin_nanf("");
-            out[s * 8 + q] = float4{nan, nan, nan, nan};
-            continue;
+            r = float4{nan, nan, nan, nan};
         }
-        const f3 p = load3(pts, s);
-        const float4 *b = data + cell * 8 + q;
-        auto D = [&](int64_t off) { return b[off * 8]; };
-        const float4 a00 = lerp4(D(0), D(1), p.x);
-        const float4 a01 = lerp4(D(R), D(R + 1), p.x);
-        const float4 a0 = lerp4(a00, a01, p.y);
-        const float4 a10 = lerp4(D(RR), D(RR + 1), p.x);
-        const float4 a11 = lerp4(D(RR + R), D(RR + R + 1), p.x);
-        const float4 a1 = lerp4(a10, a11, p.y);
-        out[s * 8 + q] = lerp4(a0, a1, p.z);
+        if (si < n) out[si * 8 + q] = r;
     }
 }
 
@@ -380,7 +395,7 @@ int qsdvr_grid_interpolation_forward(const float *data_grid, const float *point,
     hipStream_t s = as_stream(stream);
     const bool aligned16 = ((uintptr_t)data_grid % 16 == 0) && ((uintptr_t)out % 16 == 0);
     if (channels == 32 && aligned16) {
-        constexpr int kIter = 4;  // samples per wave: 8 kIter
+        constexpr int kIter = 2;  // samples per wave: 8 kIter
         const int64_t waves = (n + 8 * kIter - 1) / (8 * kIter);
         interp_fwd32_kernel<kIter><<<grid_1d(waves * kWave, 256), 256, 0, s>>>(
             reinterpret_cast<const float4 *>(data_grid), point, index, reinterpret_cast<float4 *>(out),

@@ -164,7 +164,9 @@ __device__ __forceinline__ void trilinear_weights(f3 p, float w[8]) {
 }
 
 __device__ __forceinline__ float lerp_ref(float a, float b, float w) {
-    return (1.0f - w) * a + b * w;  // render_torch.py:170-172
+    // render_torch.py:170-172, with its one rounding of the sum fixed (an
+    // fma of b w onto the rounded (1 - w) a) rather than left to contraction
+    return __builtin_fmaf(b, w, (1.0f - w) * a);
 }
 
 }  // namespace qsdvr
