@@ -131,10 +131,15 @@ def main():
     if args.copy_bytes > 0:
         src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
-        for _ in range(3):
+        ce = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ce[0].record()
+        for i in range(3):
             dst.copy_(src)
+            ce[i + 1].record()
         torch.cuda.synchronize()
-        log(f"calibration copy: {args.copy_bytes} bytes read + written, 3 launches")
+        copy_ms = min(ce[i].elapsed_time(ce[i + 1]) for i in range(3))
+        log(f"calibration copy: {args.copy_bytes} bytes read + written, 3 launches; fastest "
+            f"{copy_ms:.3f} ms = {2 * args.copy_bytes / copy_ms / 1e6:.0f} GB/s (read + write)")
         del src, dst
     G = grid.requires_grad_(True)
     D = data.requires_grad_(True)
