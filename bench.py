@@ -196,18 +196,26 @@ def main():
             out = hr.render_lists(G2, D2, inp, w.s)
             ((out - targets[i]) ** 2).mean().backward()
 
-        op_step(0)
+        # untimed warm-up over the timed poses themselves: the caching
+        # allocator then holds blocks for each pose's sample count (a pose
+        # with more samples than any before it otherwise pays device
+        # allocations inside the timed step)
+        for i in range(args.op_steps + 1):
+            op_step(i)
         torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.op_steps + 1)]
+        ev[0].record()
         for i in range(args.op_steps):
             op_step(1 + i)
-        e1.record()
+            ev[i + 1].record()
         torch.cuda.synchronize()
-        op_ms = e0.elapsed_time(e1) / args.op_steps
+        op_each = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.op_steps)]
+        op_ms = ev[0].elapsed_time(ev[-1]) / args.op_steps
         op_rays = P / (op_ms / 1e3)
         op_renders = sum(int(cam_objs[1 + i].args.renderCount) for i in range(args.op_steps))
-        log(f"op-surface: {op_ms:.3f} ms/step, {op_rays:.4g} rays/s")
+        log(f"op-surface: {op_ms:.3f} ms/step, {op_rays:.4g} rays/s; steps "
+            f"{[round(t, 2) for t in op_each]}, render samples "
+            f"{[int(cam_objs[1 + i].args.renderCount) for i in range(args.op_steps)]}")
         del G2, D2, cam_objs
         torch.cuda.empty_cache()
 
