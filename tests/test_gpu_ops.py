@@ -271,6 +271,16 @@ def test_sorted_interpolation_backward_matches_oracle(monkeypatch, C):
     qsdvr.GridInterpolationBackward(gd, pts.cuda(), idx.cuda(), up.cuda(), R + 1)
     torch.cuda.synchronize()
     assert_close(gd, d64.grad, BWD, f"sorted GridInterpolationBackward (C={C})")
+    if C == 32:
+        # upstream rows and gradient grid 4 B off 16-B alignment: the per-channel
+        # run kernel instead of the 16-B-quad one, same sums to tolerance
+        up_m = torch.empty(n * C + 1, device="cuda")[1:].view(n, C)
+        up_m.copy_(up.cuda())
+        gd_m = torch.zeros(((R + 1) ** 3) * C + 1, device="cuda")[1:].view((R + 1) ** 3, C)
+        assert up_m.data_ptr() % 16 != 0 and gd_m.data_ptr() % 16 != 0
+        qsdvr.GridInterpolationBackward(gd_m, pts.cuda(), idx.cuda(), up_m, R + 1)
+        torch.cuda.synchronize()
+        assert_close(gd_m, d64.grad, BWD, "sorted GridInterpolationBackward, unaligned rows")
 
 
 def test_normal_backward_reuses_the_interpolation_sort(monkeypatch):
