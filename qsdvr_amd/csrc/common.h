@@ -47,14 +47,18 @@ inline unsigned grid_1d(int64_t n, int block) {
 
 constexpr int kWave = 64;
 
-// Workspace carving: 256-byte aligned slices out of one caller buffer.
+// Workspace carving: slices at 256-byte aligned addresses out of one caller
+// buffer, whatever the buffer's own alignment (the size queries add 256
+// bytes for the first slice's padding; with base == nullptr `used` is the
+// size from an aligned base).
 struct Carver {
     char *base;
     size_t cap;
     size_t used = 0;
     template <typename T>
     T *take(size_t count) {
-        size_t off = (used + 255) & ~size_t(255);
+        const uintptr_t b0 = reinterpret_cast<uintptr_t>(base);
+        size_t off = (size_t)(((b0 + used + 255) & ~uintptr_t(255)) - b0);
         size_t bytes = count * sizeof(T);
         used = off + bytes;
         if (base == nullptr || used > cap) return nullptr;

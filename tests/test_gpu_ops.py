@@ -341,3 +341,43 @@ def test_normal_backward_reuses_the_interpolation_sort(monkeypatch):
     qsdvr.SampleNormalBackward(up_n, grid, rp.clone(), ri, gn)        # other point tensor
     assert torch.equal(gn, own)
     assert "qsdvr_sample_normal_backward_presorted" not in names
+
+
+def test_sorted_scatters_accept_an_unaligned_workspace():
+    """The C ABI's sorted scatters take a caller workspace of the queried size
+    at any address: its slices are placed at 256-B aligned addresses inside
+    it, so a workspace 4 B off alignment gives the same bits as an aligned
+    one (GridInterpolationBackward, the presorted SampleNormalBackward on its
+    sort, SampleSDFBackward)."""
+    import ctypes as C
+    from qsdvr_amd import _lib
+    from qsdvr_amd.qsdvr import _ptr, _stream
+    R = 16
+    g = torch.Generator(device="cuda").manual_seed(5)
+    n = 50000
+    idx = torch.randint(0, R ** 3, (n,), generator=g, device="cuda", dtype=torch.int32)
+    pts = torch.rand((n, 3), generator=g, device="cuda")
+    up = torch.randn((n, 32), generator=g, device="cuda")
+    upn = torch.randn((n, 3), generator=g, device="cuda")
+    ups = torch.randn((n,), generator=g, device="cuda")
+    grid = torch.randn((R, R, R, 7), generator=g, device="cuda")
+    nb = int(_lib.fn("qsdvr_sorted_scatter_workspace_size")(n, R ** 3))
+    results = []
+    for shift in (0, 4):
+        raw = torch.empty(nb + 512, dtype=torch.uint8, device="cuda")
+        ws = raw[(-raw.data_ptr()) % 256 + shift:][:nb]
+        assert ws.data_ptr() % 256 == shift and ws.numel() == nb
+        gd = torch.zeros(((R + 1) ** 3, 32), device="cuda")
+        gn = torch.zeros_like(grid)
+        gs = torch.zeros_like(grid)
+        _lib.call("qsdvr_grid_interpolation_backward_sorted", _ptr(gd), _ptr(pts), _ptr(idx),
+                  _ptr(up), n, R + 1, 32, _ptr(ws), nb, _stream())
+        _lib.call("qsdvr_sample_normal_backward_presorted", _ptr(upn), _ptr(grid), _ptr(gn), n,
+                  R, _ptr(ws), nb, _stream())
+        _lib.call("qsdvr_sample_sdf_backward_sorted", _ptr(ups), _ptr(grid), _ptr(pts), _ptr(idx),
+                  _ptr(gs), n, R ** 3, float(R), _ptr(ws), nb, _stream())
+        torch.cuda.synchronize()
+        results.append((gd, gn, gs))
+    for a, b in zip(*results):
+        assert torch.equal(a, b)
+    assert results[0][0].abs().sum() > 0 and results[0][1].abs().sum() > 0
