@@ -143,9 +143,12 @@ def _op(fn):
 
     @functools.wraps(fn)
     def wrapped(*args, **kwargs):
-        global _calls
+        global _calls, _render_sort
         _calls += 1
-        return fn(*args, **kwargs)
+        result = fn(*args, **kwargs)
+        if _render_sort is not None and _render_sort[-1] != _calls:
+            _render_sort = None  # an earlier call's sort: no later call may use it
+        return result
     return wrapped
 
 
@@ -396,7 +399,8 @@ def quadric_backward_path(n, sdfGrid):
 # references to the point and index tensors, their version counters (an
 # in-place change invalidates it), the size and resolution, the stream and
 # the op-call count: only the very next op call, if it is
-# SampleNormalBackward on those tensors, takes it.  (The reused order sums a
+# SampleNormalBackward on those tensors, takes it, and any other next call
+# drops it (freeing the workspace).  (The reused order sums a
 # cell's samples in a different tree than SampleNormalBackward's own sort,
 # so the bits then depend on the preceding call -- the same sequence of
 # calls still gives the same bits.)

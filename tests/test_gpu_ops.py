@@ -328,6 +328,7 @@ def test_normal_backward_reuses_the_interpolation_sort(monkeypatch):
     names.clear()
     interp()
     qsdvr.SampleNormalForward(torch.empty_like(up_n), grid, rp, ri)   # an intervening call
+    assert qsdvr._render_sort is None                                   # its workspace freed
     again = normal()
     assert "qsdvr_sample_normal_backward_presorted" not in names
     assert torch.equal(again, own)
