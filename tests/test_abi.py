@@ -108,3 +108,14 @@ def test_sorted_scatters_validate_on_the_host():
     assert st == -4 and b"too small" in lib.qsdvr_last_error()
     assert lib.qsdvr_sample_normal_backward_sorted(None, None, None, None, None, 0, 512, None, 0,
                                                    None) == 0
+    # the presorted SampleNormalBackward: an even resolution, its workspace
+    # size checked against GridInterpolationBackward's, nothing to do for n = 0
+    assert lib.qsdvr_sample_normal_backward_presorted(None, None, None, 0, 8, None, 0, None) == 0
+    st = lib.qsdvr_sample_normal_backward_presorted(dummy, dummy, dummy, 100, 9, dummy, 1 << 20,
+                                                    None)
+    assert st == -1 and b"even" in lib.qsdvr_last_error()
+    st = lib.qsdvr_sample_normal_backward_presorted(dummy, dummy, dummy, 100, 8, dummy, 16, None)
+    assert st == -4 and b"too small" in lib.qsdvr_last_error()
+    st = lib.qsdvr_sample_normal_backward_presorted(None, dummy, dummy, 100, 8, dummy, 1 << 20,
+                                                    None)
+    assert st == -1 and b"null" in lib.qsdvr_last_error()
