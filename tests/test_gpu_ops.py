@@ -70,6 +70,32 @@ def test_sample_normal(wl):
     assert_close(gg.reshape(-1, 7), g64.grad.reshape(-1, 7), BWD, "SampleNormal backward")
 
 
+def test_presorted_sample_normal_backward_matches_oracle(wl, monkeypatch):
+    """SampleNormalBackward summed on GridInterpolationBackward's sort of the
+    same samples (the reference's backward order) against the oracle's
+    adjoint of SampleNormal (float64 autograd of render_torch.py:91-100)."""
+    from qsdvr_amd import _lib
+    d = wl["dev"]
+    R = wl["w"].R
+    grid, data = wl["grid"].cuda(), wl["data"].cuda()
+    pts, idx = d["renderPointList"], d["renderIndexList"]
+    n = pts.shape[0]
+    g64 = _f64(wl["grid"])
+    ref = orc.sample_normal(g64, torch.as_tensor(wl["smp"]["renderPointList"]).double(),
+                            torch.as_tensor(wl["smp"]["renderIndexList"]))
+    up = torch.randn((n, 3), generator=torch.Generator().manual_seed(12))
+    (ref * up.double()).sum().backward()
+    names = []
+    real_call = _lib.call
+    monkeypatch.setattr(_lib, "call", lambda name, *a: (names.append(name), real_call(name, *a)))
+    qsdvr.GridInterpolationBackward(torch.zeros_like(data), pts, idx,
+                                    torch.randn((n, data.shape[-1]), device="cuda"), R + 1)
+    gg = torch.zeros_like(grid)
+    qsdvr.SampleNormalBackward(up.cuda(), grid, pts, idx, gg)
+    assert names[-1] == "qsdvr_sample_normal_backward_presorted"
+    assert_close(gg.reshape(-1, 7), g64.grad.reshape(-1, 7), BWD, "presorted SampleNormal backward")
+
+
 def test_grid_interpolation(wl):
     R = wl["w"].R
     d = wl["dev"]
