@@ -378,6 +378,40 @@ int qsdvr_fused_backward_listed(const qsdvr_camera *cams, int32_t n_cams,
                                 float *data_grid_grad, int32_t scatter_mode,
                                 qsdvr_stream stream);
 
+/* Cost-ordered tile adjoint of the fused renderer (no reference counterpart;
+ * the reference has no fused renderer).  The tile adjoint's time per 8x8
+ * tile varies about 4x across a frame, and a launch in a fixed order leaves
+ * its heaviest tiles to finish last.  qsdvr_fused_forward_costed is
+ * qsdvr_fused_forward on whole frames (or qsdvr_fused_forward_listed, when
+ * ray_state and tlist are given) that also writes each tile's expected
+ * backward cost to tile_cost[n_cams * ceil(W/8) * ceil(H/8)] (index =
+ * camera * tiles + row-major tile).  qsdvr_fused_backward_ordered is the
+ * tile-mode qsdvr_fused_backward (or _listed) that sorts the tiles by that
+ * cost in its workspace (qsdvr_fused_order_workspace_size bytes) and runs
+ * them heaviest first: the same gradient sums, with its float atomics in
+ * another order. */
+size_t qsdvr_fused_order_workspace_size(int32_t n_cams, int32_t width, int32_t height);
+int qsdvr_fused_forward_costed(const qsdvr_camera *cams, int32_t n_cams,
+                               int32_t width, int32_t height,
+                               const float *ray_state, const float *tlist,
+                               int32_t K, const float *sdf_grid,
+                               const float *data_grid, int32_t reso,
+                               int32_t channels, float interval,
+                               float logistic_coef, float *color, float *depth,
+                               uint32_t *tile_cost, qsdvr_stream stream);
+int qsdvr_fused_backward_ordered(const qsdvr_camera *cams, int32_t n_cams,
+                                 int32_t width, int32_t height,
+                                 const float *ray_state, const float *tlist,
+                                 int32_t K, const float *sdf_grid,
+                                 const float *data_grid, int32_t reso,
+                                 int32_t channels, float interval,
+                                 float logistic_coef, const float *color,
+                                 const float *depth, const float *grad_color,
+                                 const float *grad_depth, float *sdf_grid_grad,
+                                 float *data_grid_grad, const uint32_t *tile_cost,
+                                 void *workspace, size_t workspace_bytes,
+                                 qsdvr_stream stream);
+
 /* Deterministic adjoint (sorted-segment gradient scatter).  No reference
  * counterpart: the reference's backward kernels add every sample's gradient
  * with float atomics (kernel/render.cu:121-152, kernel/surface.cu:74-146),

@@ -74,6 +74,11 @@ SIGNATURES = {
                                  C.c_int),
     "qsdvr_ray_state": ([P, I32, I32, I32, P, I64, F, P, P], C.c_int),
     "qsdvr_plain_bins": ([P, I64, P, I32, F, F, P, P], C.c_int),
+    "qsdvr_fused_order_workspace_size": ([I32, I32, I32], SZ),
+    "qsdvr_fused_forward_costed": ([P, I32, I32, I32, P, P, I32, P, P, I32, I32, F, F, P, P, P, P],
+                                   C.c_int),
+    "qsdvr_fused_backward_ordered": ([P, I32, I32, I32, P, P, I32, P, P, I32, I32, F, F, P, P, P, P,
+                                      P, P, P, P, SZ, P], C.c_int),
     "qsdvr_fused_forward_listed": ([P, I32, I32, I32, P, I64, P, P, I32, P, P, I32, I32, F, F,
                                     P, P, P], C.c_int),
     "qsdvr_fused_backward_listed": ([P, I32, I32, I32, P, I64, P, P, I32, P, P, I32, I32, F, F,

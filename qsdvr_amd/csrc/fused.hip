@@ -28,6 +28,7 @@
 #include <limits.h>
 
 #include <algorithm>
+#include <rocprim/device/device_radix_sort.hpp>
 
 namespace qsdvr {
 
@@ -64,21 +65,27 @@ __device__ __forceinline__ RaySetup setup_ray(const qsdvr_camera *__restrict__ c
                                               int64_t n_rays, float interval,
                                               const float4 *__restrict__ ray_state = nullptr,
                                               const float *__restrict__ tlist = nullptr,
-                                              int K = 0) {
+                                              int K = 0,
+                                              const uint32_t *__restrict__ order = nullptr) {
     RaySetup r;
     r.out = -1;
     r.renders = 0;
     r.tl = nullptr;
     const int lane = threadIdx.x % kWave;
-    const int64_t wave = (int64_t)logical_block() * WPB + threadIdx.x / kWave;
+    // with a tile order, launch block b takes tile order[b] (blocks are dealt
+    // round-robin to the XCDs, so each XCD gets every 8th tile of the order)
+    const int64_t wave = order != nullptr ? (int64_t)blockIdx.x * WPB + threadIdx.x / kWave
+                                          : (int64_t)logical_block() * WPB + threadIdx.x / kWave;
     int cam = 0;
     uint32_t pix = 0;
     if (ray_pixel == nullptr) {
         const int64_t tx = (W + 7) / 8, ty = (H + 7) / 8, tiles = tx * ty;
         if (wave >= tiles * n_cams) return r;
-        cam = (int)(wave / tiles);
-        int64_t t = wave - (int64_t)cam * tiles;
-        if (SCATTER && tiles % kTileStride != 0) t = (t * kTileStride) % tiles;
+        const int64_t g = order != nullptr ? (int64_t)order[wave] : wave;
+        if (g >= tiles * n_cams) return r;
+        cam = (int)(g / tiles);
+        int64_t t = g - (int64_t)cam * tiles;
+        if (SCATTER && order == nullptr && tiles % kTileStride != 0) t = (t * kTileStride) % tiles;
         uint32_t px = (uint32_t)(t % tx) * 8 + (lane % 8);
         uint32_t py = (uint32_t)(t / tx) * 8 + (lane / 8);
         if (px >= W || py >= H) return r;
@@ -266,17 +273,19 @@ __device__ __forceinline__ void box_interp(const BoxView &bx, const float w[8], 
 
 // Stage the box of the cells of the lanes with ``need`` set (their cell
 // ``rp``): returns the lane's view of it, or a null view (gather from global
-// memory) when the box has more than kBoxV vertices.  Called by the whole
-// wave with every lane active.
+// memory) when the box has more than kBoxV vertices, and the box's vertex
+// count in nv_out.  Called by the whole wave with every lane active.
 template <int C>
 __device__ __forceinline__ BoxView stage_box(float4 *box, const float *__restrict__ data, int R,
-                                             bool need, const CellPoint &rp, int lane) {
+                                             bool need, const CellPoint &rp, int lane,
+                                             int &nv_out) {
     constexpr int Q = C / 4, QP = Q + 1;
     const int lx = wave_min_s(need ? rp.ix : INT_MAX), hx = wave_max_s(need ? rp.ix : INT_MIN);
     const int ly = wave_min_s(need ? rp.iy : INT_MAX), hy = wave_max_s(need ? rp.iy : INT_MIN);
     const int lz = wave_min_s(need ? rp.iz : INT_MAX), hz = wave_max_s(need ? rp.iz : INT_MIN);
     const int ex = hx - lx + 2, ey = hy - ly + 2, ez = hz - lz + 2;
     const int sxy = ex * ey, nv = sxy * ez;
+    nv_out = nv;
     if (nv > kBoxV) return BoxView{nullptr, 0, 0, 0};
     // box vertex v = vx + vy ex + vz sxy (v < 64, sxy >= 4, ex >= 2): the
     // quotients from float reciprocals are exact, since (v + 0.5) / sxy lies
@@ -470,7 +479,7 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
     int64_t n_rays, const float *__restrict__ qgrid, const float *__restrict__ data, int R,
     float interval, float s, float *__restrict__ color, float *__restrict__ depth,
     const float4 *__restrict__ ray_state, const float *__restrict__ tlist, int K,
-    float *__restrict__ bins) {
+    float *__restrict__ bins, uint32_t *__restrict__ tile_cost) {
     constexpr int C = 3 * NC + 5;
     extern __shared__ float4 fwd_lds[];
     RaySetup ray = setup_ray(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state,
@@ -507,6 +516,11 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
     // in a register and stored once (the caller zeroes the bins).
     int bin = 0;
     float bin_sum = 0.0f;
+    // The tile's expected backward cost, for qsdvr_fused_backward_ordered:
+    // 64 per step with an appearance sample, plus the step's box vertices,
+    // plus 4 times them when the box overflows (the adjoint then needs more
+    // than one round).  Fitted to measured per-tile backward times.
+    uint32_t cost = 0;
     for (int j = 0; j < steps; ++j) {
         const bool act = live && j < (int)ray.renders;
         float a = 0.0f, wt = 0.0f, e1 = 0.0f;
@@ -529,6 +543,7 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
         // appearance gather and shading are skipped.
         const bool need = wt != 0.0f;
         if (__ballot(need) != 0) {
+            cost += 64;
             CellPoint rp;
             rp.ix = rp.iy = rp.iz = 0;
             rp.cell = 0;
@@ -539,7 +554,11 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
             }
             BoxView bx{nullptr, 0, 0, 0};
             if constexpr (C == 32) {
-                if (stage) bx = stage_box<C>(box, data, R, need, rp, lane);
+                if (stage) {
+                    int nv = 0;
+                    bx = stage_box<C>(box, data, R, need, rp, lane, nv);
+                    cost += (uint32_t)nv + (nv > kBoxV ? 4u * (uint32_t)nv : 0u);
+                }
             }
             if (need) {
                 float w[8], d[C];
@@ -566,6 +585,10 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
             e0 = e1;
             tw.advance(ray, j);
         }
+    }
+    if (tile_cost != nullptr && ray_pixel == nullptr && lane == 0) {
+        const int64_t tw_ = (int64_t)logical_block() * kWavesPerBlock + threadIdx.x / kWave;
+        if (tw_ < (int64_t)((W + 7) / 8) * ((H + 7) / 8) * n_cams) tile_cost[tw_] = cost;
     }
     if (ray.out < 0) return;
     if constexpr (BINS) {
@@ -1232,7 +1255,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
     float interval, float s, const float *__restrict__ color, const float *__restrict__ depth,
     const float *__restrict__ gcolor, const float *__restrict__ gdepth,
     float *__restrict__ gq, float *__restrict__ gdata, const float4 *__restrict__ ray_state,
-    const float *__restrict__ tlist, int K) {
+    const float *__restrict__ tlist, int K, const uint32_t *__restrict__ tile_order) {
     constexpr int C = 3 * NC + 5;
     constexpr int CP = tile_channels<NC>();
     extern __shared__ float4 lds4[];
@@ -1245,9 +1268,10 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
 
     // tiles in scattered order: measured 7% faster here (12% on listed
     // samples) than raster order; the forward keeps raster order, whose
-    // neighbouring tiles share their staged rows in L2
+    // neighbouring tiles share their staged rows in L2.  With a tile order
+    // (qsdvr_fused_backward_ordered) the tiles run in that order instead.
     RaySetup ray = setup_ray<1, true>(cams, n_cams, W, H, ray_pixel, n_rays, interval, ray_state,
-                                      LISTED ? tlist : nullptr, K);
+                                      LISTED ? tlist : nullptr, K, tile_order);
     if constexpr (!LISTED) ray.tl = nullptr;
     const bool live = ray.out >= 0 && ray.renders > 0;
     const int steps = wave_max(live ? (int)ray.renders + 1 : 0);
@@ -1689,7 +1713,8 @@ static int fused_forward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t 
                               const float *ray_state, const float *tlist, int32_t K,
                               const float *sdf_grid, const float *data_grid, int32_t reso,
                               int32_t channels, float interval, float logistic_coef,
-                              float *color, float *depth, float *bins, hipStream_t s) {
+                              float *color, float *depth, float *bins, hipStream_t s,
+                              uint32_t *tile_cost = nullptr) {
     int st = fused_check(cams, n_cams, width, height, ray_pixel, n_rays, sdf_grid, data_grid,
                          reso, channels, interval);
     if (st != QSDVR_OK) return st;
@@ -1703,7 +1728,7 @@ static int fused_forward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t 
 #define QSDVR_FWD(NC, BINS, LISTED)                                                          \
     fused_fwd_kernel<NC, BINS, LISTED><<<blocks, kFusedBlock, fwd_lds_bytes<NC>(), s>>>(     \
         cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid, data_grid, \
-        reso, interval, logistic_coef, color, depth, rs, tlist, K, bins)
+        reso, interval, logistic_coef, color, depth, rs, tlist, K, bins, tile_cost)
 #define QSDVR_FWD_NC(NC)                          \
     do {                                          \
         if (tlist != nullptr) {                   \
@@ -1729,7 +1754,8 @@ static int fused_backward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t
                                int32_t channels, float interval, float logistic_coef,
                                const float *color, const float *depth, const float *grad_color,
                                const float *grad_depth, float *sdf_grid_grad,
-                               float *data_grid_grad, int32_t scatter_mode, hipStream_t s) {
+                               float *data_grid_grad, int32_t scatter_mode, hipStream_t s,
+                               const uint32_t *tile_order = nullptr) {
     int st = fused_check(cams, n_cams, width, height, ray_pixel, n_rays, sdf_grid, data_grid,
                          reso, channels, interval);
     if (st != QSDVR_OK) return st;
@@ -1765,7 +1791,7 @@ static int fused_backward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t
     fused_bwd_tile_kernel<NC, LISTED><<<blocks, kWave, tile_lds_bytes<NC>(), s>>>(           \
         cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid,        \
         data_grid, reso, interval, logistic_coef, color, dsave, grad_color, grad_depth,      \
-        sdf_grid_grad, data_grid_grad, rs, tlist, K)
+        sdf_grid_grad, data_grid_grad, rs, tlist, K, tile_order)
     if (channels == 32) {
         if (tlist != nullptr) QSDVR_TILE(9, true);
         else QSDVR_TILE(9, false);
@@ -1869,6 +1895,107 @@ int qsdvr_fused_backward_listed(const qsdvr_camera *cams, int32_t n_cams, int32_
                                K, sdf_grid, data_grid, reso, channels, interval, logistic_coef,
                                color, depth, grad_color, grad_depth, sdf_grid_grad,
                                data_grid_grad, scatter_mode, as_stream(stream));
+}
+
+// ------------------------------------------ cost-ordered tile backward ----
+// The tile adjoint's time per 8x8 tile varies about 4x across a frame (tiles
+// whose appearance boxes overflow need several rounds per step), and a launch
+// in a fixed order leaves its heaviest tiles to finish last.  The costed
+// forward records each tile's expected backward cost; the ordered backward
+// sorts the tiles by it (heaviest first) and runs them in that order.
+
+}  // extern "C"
+
+namespace {
+
+__global__ void iota_kernel(uint32_t *__restrict__ v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (uint32_t)i;
+}
+
+size_t order_temp_bytes(int64_t n) {
+    size_t t = 0;
+    (void)rocprim::radix_sort_pairs_desc(nullptr, t, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                         (const uint32_t *)nullptr, (uint32_t *)nullptr, (size_t)n,
+                                         0u, 32u, (hipStream_t)0);
+    return t;
+}
+
+struct OrderBuffers {
+    uint32_t *keys_out, *ids, *order;
+    void *temp;
+    size_t temp_bytes;
+};
+
+size_t order_carve(void *ws, size_t bytes, int64_t n, OrderBuffers &b) {
+    Carver c{static_cast<char *>(ws), bytes};
+    b.keys_out = c.take<uint32_t>(n);
+    b.ids = c.take<uint32_t>(n);
+    b.order = c.take<uint32_t>(n);
+    b.temp_bytes = order_temp_bytes(n);
+    b.temp = c.take<char>(b.temp_bytes);
+    return c.used + 256;
+}
+
+int64_t frame_tiles(int32_t n_cams, int32_t width, int32_t height) {
+    return (int64_t)((width + 7) / 8) * ((height + 7) / 8) * n_cams;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t qsdvr_fused_order_workspace_size(int32_t n_cams, int32_t width, int32_t height) {
+    if (n_cams <= 0 || width <= 0 || height <= 0) return 0;
+    OrderBuffers b;
+    return order_carve(nullptr, 0, frame_tiles(n_cams, width, height), b);
+}
+
+int qsdvr_fused_forward_costed(const qsdvr_camera *cams, int32_t n_cams, int32_t width,
+                               int32_t height, const float *ray_state, const float *tlist,
+                               int32_t K, const float *sdf_grid, const float *data_grid,
+                               int32_t reso, int32_t channels, float interval,
+                               float logistic_coef, float *color, float *depth,
+                               uint32_t *tile_cost, qsdvr_stream stream) {
+    QSDVR_REQUIRE(tile_cost != nullptr, "qsdvr_fused_forward_costed: null tile_cost");
+    QSDVR_REQUIRE(n_cams > 0 && width > 0 && height > 0,
+                  "qsdvr_fused_forward_costed: bad frame size");
+    return fused_forward_impl(cams, n_cams, width, height, nullptr, (int64_t)n_cams * width * height,
+                              ray_state, tlist, K, sdf_grid, data_grid, reso, channels, interval,
+                              logistic_coef, color, depth, nullptr, as_stream(stream), tile_cost);
+}
+
+int qsdvr_fused_backward_ordered(const qsdvr_camera *cams, int32_t n_cams, int32_t width,
+                                 int32_t height, const float *ray_state, const float *tlist,
+                                 int32_t K, const float *sdf_grid, const float *data_grid,
+                                 int32_t reso, int32_t channels, float interval,
+                                 float logistic_coef, const float *color, const float *depth,
+                                 const float *grad_color, const float *grad_depth,
+                                 float *sdf_grid_grad, float *data_grid_grad,
+                                 const uint32_t *tile_cost, void *workspace,
+                                 size_t workspace_bytes, qsdvr_stream stream) {
+    QSDVR_REQUIRE(tile_cost != nullptr, "qsdvr_fused_backward_ordered: null tile_cost");
+    QSDVR_REQUIRE(n_cams > 0 && width > 0 && height > 0,
+                  "qsdvr_fused_backward_ordered: bad frame size");
+    const int64_t n = frame_tiles(n_cams, width, height);
+    OrderBuffers b;
+    const size_t need = order_carve(nullptr, 0, n, b);
+    if (workspace == nullptr || workspace_bytes < need) {
+        set_error("qsdvr_fused_backward_ordered: workspace of %zu bytes is too small (%zu needed)",
+                  workspace_bytes, need);
+        return QSDVR_EWORKSPACE;
+    }
+    order_carve(workspace, workspace_bytes, n, b);
+    hipStream_t s = as_stream(stream);
+    iota_kernel<<<grid_1d(n, 256), 256, 0, s>>>(b.ids, n);
+    QSDVR_LAUNCHED();
+    size_t tb = b.temp_bytes;
+    QSDVR_HIP(rocprim::radix_sort_pairs_desc(b.temp, tb, tile_cost, b.keys_out, b.ids, b.order,
+                                             (size_t)n, 0u, 32u, s));
+    return fused_backward_impl(cams, n_cams, width, height, nullptr, (int64_t)n_cams * width * height,
+                               ray_state, tlist, K, sdf_grid, data_grid, reso, channels, interval,
+                               logistic_coef, color, depth, grad_color, grad_depth, sdf_grid_grad,
+                               data_grid_grad, 0, s, b.order);
 }
 
 size_t qsdvr_fused_det_workspace_size(int64_t n_rays, int64_t sample_capacity, int32_t channels,

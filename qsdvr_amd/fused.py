@@ -23,9 +23,24 @@ SCATTER_TILE = 0     # tile-reduced atomics
 SCATTER_ATOMIC = 1   # one atomic per sample and channel
 SCATTER_DET = 2      # sorted-segment sums, bitwise reproducible
 
+# Whole-frame tile-mode renders record each 8x8 tile's expected backward
+# cost in the forward, and the adjoint then runs the tiles heaviest first
+# (qsdvr_fused_forward_costed / qsdvr_fused_backward_ordered): a fixed launch
+# order leaves the heaviest tiles to finish last.  False: the launch order.
+TILE_ORDER = True
+
 # When set to a list, every fused kernel launch appends
 # (name, start_event, end_event) recorded on the launching stream.
 TIMERS = None
+
+
+def _tile_count(n_cams, width, height):
+    return n_cams * ((width + 7) // 8) * ((height + 7) // 8)
+
+
+def _order_workspace(n_cams, width, height, device):
+    nb = int(_lib.fn("qsdvr_fused_order_workspace_size")(n_cams, width, height))
+    return torch.empty(max(nb, 1), dtype=torch.uint8, device=device), nb
 
 
 class _timed:
@@ -79,11 +94,19 @@ class FusedRender(autograd.Function):
         dev = sdf_grid.device
         color = torch.empty((n_rays, 3), dtype=torch.float32, device=dev)
         depth = torch.empty((n_rays,), dtype=torch.float32, device=dev)
+        cost = None
         with _timed("fused_forward"):
-            _lib.call("qsdvr_fused_forward", _ptr(cam_buf), n_cams, width, height,
-                      _ptr(ray_pixel), n_rays, _ptr(sdf_grid), _ptr(data_grid), R, Cc,
-                      float(interval), float(s), _ptr(color), _ptr(depth), _stream())
-        ctx.save_for_backward(sdf_grid, data_grid, cam_buf, ray_pixel, color, depth)
+            if TILE_ORDER and ray_pixel is None and scatter_mode == SCATTER_TILE:
+                cost = torch.empty(_tile_count(n_cams, width, height), dtype=torch.int32,
+                                   device=dev)
+                _lib.call("qsdvr_fused_forward_costed", _ptr(cam_buf), n_cams, width, height,
+                          None, None, 0, _ptr(sdf_grid), _ptr(data_grid), R, Cc, float(interval),
+                          float(s), _ptr(color), _ptr(depth), _ptr(cost), _stream())
+            else:
+                _lib.call("qsdvr_fused_forward", _ptr(cam_buf), n_cams, width, height,
+                          _ptr(ray_pixel), n_rays, _ptr(sdf_grid), _ptr(data_grid), R, Cc,
+                          float(interval), float(s), _ptr(color), _ptr(depth), _stream())
+        ctx.save_for_backward(sdf_grid, data_grid, cam_buf, ray_pixel, color, depth, cost)
         ctx.args = (n_cams, width, height, n_rays, R, Cc, float(interval), float(s),
                     int(scatter_mode))
         ctx.set_materialize_grads(False)
@@ -91,7 +114,7 @@ class FusedRender(autograd.Function):
 
     @staticmethod
     def backward(ctx: Any, gcolor, gdepth):
-        sdf_grid, data_grid, cam_buf, ray_pixel, color, depth = ctx.saved_tensors
+        sdf_grid, data_grid, cam_buf, ray_pixel, color, depth, cost = ctx.saved_tensors
         n_cams, width, height, n_rays, R, Cc, interval, s, mode = ctx.args
         dev = sdf_grid.device
         gq = torch.zeros_like(sdf_grid)
@@ -111,10 +134,17 @@ class FusedRender(autograd.Function):
                              data_grid, interval, s, color, depth, gcolor, gdep, gq, gd)
             return gq, gd, None, None, None, None, None, None, None, None
         with _timed("fused_backward"):
-            _lib.call("qsdvr_fused_backward", _ptr(cam_buf), n_cams, width, height,
-                      _ptr(ray_pixel), n_rays, _ptr(sdf_grid), _ptr(data_grid), R, Cc, interval,
-                      s, _ptr(color), _ptr(depth), _ptr(gcolor), _ptr(gdep), _ptr(gq), _ptr(gd),
-                      mode, _stream())
+            if cost is not None:
+                ws, nb = _order_workspace(n_cams, width, height, dev)
+                _lib.call("qsdvr_fused_backward_ordered", _ptr(cam_buf), n_cams, width, height,
+                          None, None, 0, _ptr(sdf_grid), _ptr(data_grid), R, Cc, interval, s,
+                          _ptr(color), _ptr(depth), _ptr(gcolor), _ptr(gdep), _ptr(gq), _ptr(gd),
+                          _ptr(cost), _ptr(ws), nb, _stream())
+            else:
+                _lib.call("qsdvr_fused_backward", _ptr(cam_buf), n_cams, width, height,
+                          _ptr(ray_pixel), n_rays, _ptr(sdf_grid), _ptr(data_grid), R, Cc,
+                          interval, s, _ptr(color), _ptr(depth), _ptr(gcolor), _ptr(gdep),
+                          _ptr(gq), _ptr(gd), mode, _stream())
         return gq, gd, None, None, None, None, None, None, None, None
 
 

@@ -139,3 +139,43 @@ def test_degenerate_cells_keep_gradients_finite(small):
     ((out - tgt) ** 2).mean().backward()
     assert torch.isfinite(out).all()
     assert torch.isfinite(G.grad).all() and torch.isfinite(D.grad).all()
+
+
+def test_cost_ordered_backward_matches_launch_order(monkeypatch):
+    """The tile adjoint run heaviest-first by the forward's per-tile cost
+    (qsdvr_fused_backward_ordered, the default for whole frames) gives the
+    launch-order adjoint's gradients (float atomics in another order: to
+    1e-4) and the same colours bit for bit, on a full 800x800 pose and on
+    two poses in one call; every tile gets a cost."""
+    from qsdvr_amd import _lib
+    from qsdvr_amd.qsdvr import _ptr, _stream
+    w = scenes.WORKLOADS["plain"]
+    cams = scenes.orbit_cameras(100, w.width, w.height)
+    grid, data, _ = scenes.build_grids(w, seed=0)
+    tg = torch.rand((2 * w.width * w.height, 3), generator=torch.Generator().manual_seed(4)).cuda()
+    for pick in ([cams[11]], [cams[7], cams[30]]):
+        out = {}
+        for order in (False, True):
+            monkeypatch.setattr(fused, "TILE_ORDER", order)
+            G = grid.cuda().requires_grad_(True)
+            D = data.cuda().requires_grad_(True)
+            c, _ = fused.render(G, D, pick, w.interval, w.s)
+            ((c - tg[:c.shape[0]]) ** 2).mean().backward()
+            out[order] = (c.detach(), G.grad, D.grad)
+        assert torch.equal(out[False][0], out[True][0])
+        assert_close(out[True][1].reshape(-1, 7), out[False][1].reshape(-1, 7), BWD,
+                     "ordered vs launch-order sdf grad")
+        assert_close(out[True][2], out[False][2], BWD, "ordered vs launch-order appearance grad")
+    # the costs themselves: every tile written, the object's tiles costlier than empty ones
+    cb = fused.camera_buffer(cams[11:12])
+    n_tiles = ((w.width + 7) // 8) * ((w.height + 7) // 8)
+    cost = torch.full((n_tiles,), -1, dtype=torch.int32, device="cuda")
+    col = torch.empty((w.width * w.height, 3), device="cuda")
+    dep = torch.empty((w.width * w.height,), device="cuda")
+    G, D = grid.cuda(), data.cuda()
+    _lib.call("qsdvr_fused_forward_costed", _ptr(cb), 1, w.width, w.height, None, None, 0, _ptr(G),
+              _ptr(D), w.R, 32, float(w.interval), float(w.s), _ptr(col), _ptr(dep), _ptr(cost),
+              _stream())
+    torch.cuda.synchronize()
+    assert (cost != -1).all() and (cost >= 0).all()
+    assert cost.max() > 64 and cost.min() == 0   # corner tiles see nothing of the object
