@@ -472,7 +472,11 @@ __device__ __forceinline__ float sdf_at(const RaySetup &ray, uint32_t j, float i
 // LISTED: the rays carry explicit sample lists (tlist); the uniform-ray
 // instantiation drops the list code and its registers.  C = 32: at most 128
 // registers per lane, 4 waves per SIMD (as many as the staging LDS allows).
-template <int NC, bool BINS, bool LISTED>
+// COST: record each tile's expected backward cost in tile_cost (frame
+// launches only).  Only the costed entry point instantiates it: at 53
+// channels the store alone moves the compiler to a schedule with fewer
+// registers that measured 47% slower.
+template <int NC, bool BINS, bool LISTED, bool COST = false>
 __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel(
     const qsdvr_camera *__restrict__ cams, int n_cams, uint32_t W, uint32_t H,
     const int32_t *__restrict__ ray_pixel,
@@ -520,7 +524,7 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
     // 64 per step with an appearance sample, plus the step's box vertices,
     // plus 4 times them when the box overflows (the adjoint then needs more
     // than one round).  Fitted to measured per-tile backward times.
-    uint32_t cost = 0;
+    [[maybe_unused]] uint32_t cost = 0;
     for (int j = 0; j < steps; ++j) {
         const bool act = live && j < (int)ray.renders;
         float a = 0.0f, wt = 0.0f, e1 = 0.0f;
@@ -543,7 +547,7 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
         // appearance gather and shading are skipped.
         const bool need = wt != 0.0f;
         if (__ballot(need) != 0) {
-            cost += 64;
+            if constexpr (COST) cost += 64;
             CellPoint rp;
             rp.ix = rp.iy = rp.iz = 0;
             rp.cell = 0;
@@ -557,7 +561,7 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
                 if (stage) {
                     int nv = 0;
                     bx = stage_box<C>(box, data, R, need, rp, lane, nv);
-                    cost += (uint32_t)nv + (nv > kBoxV ? 4u * (uint32_t)nv : 0u);
+                    if constexpr (COST) cost += (uint32_t)nv + (nv > kBoxV ? 4u * (uint32_t)nv : 0u);
                 }
             }
             if (need) {
@@ -586,9 +590,11 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
             tw.advance(ray, j);
         }
     }
-    if (tile_cost != nullptr && ray_pixel == nullptr && lane == 0) {
-        const int64_t tw_ = (int64_t)logical_block() * kWavesPerBlock + threadIdx.x / kWave;
-        if (tw_ < (int64_t)((W + 7) / 8) * ((H + 7) / 8) * n_cams) tile_cost[tw_] = cost;
+    if constexpr (COST) {
+        if (ray_pixel == nullptr && lane == 0) {
+            const int64_t tw_ = (int64_t)logical_block() * kWavesPerBlock + threadIdx.x / kWave;
+            if (tw_ < (int64_t)((W + 7) / 8) * ((H + 7) / 8) * n_cams) tile_cost[tw_] = cost;
+        }
     }
     if (ray.out < 0) return;
     if constexpr (BINS) {
@@ -1725,19 +1731,22 @@ static int fused_forward_impl(const qsdvr_camera *cams, int32_t n_cams, int32_t 
     const float4 *rs = reinterpret_cast<const float4 *>(ray_state);
     int64_t waves = fused_waves(width, height, n_cams, ray_pixel, n_rays);
     unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
-#define QSDVR_FWD(NC, BINS, LISTED)                                                          \
-    fused_fwd_kernel<NC, BINS, LISTED><<<blocks, kFusedBlock, fwd_lds_bytes<NC>(), s>>>(     \
+#define QSDVR_FWD(NC, BINS, LISTED, COST)                                                    \
+    fused_fwd_kernel<NC, BINS, LISTED, COST><<<blocks, kFusedBlock, fwd_lds_bytes<NC>(), s>>>( \
         cams, n_cams, (uint32_t)width, (uint32_t)height, ray_pixel, n_rays, sdf_grid, data_grid, \
         reso, interval, logistic_coef, color, depth, rs, tlist, K, bins, tile_cost)
-#define QSDVR_FWD_NC(NC)                          \
-    do {                                          \
-        if (tlist != nullptr) {                   \
-            if (bins) QSDVR_FWD(NC, true, true);  \
-            else QSDVR_FWD(NC, false, true);      \
-        } else {                                  \
-            if (bins) QSDVR_FWD(NC, true, false); \
-            else QSDVR_FWD(NC, false, false);     \
-        }                                         \
+#define QSDVR_FWD_NC(NC)                                  \
+    do {                                                  \
+        if (tile_cost != nullptr) {                       \
+            if (tlist != nullptr) QSDVR_FWD(NC, false, true, true); \
+            else QSDVR_FWD(NC, false, false, true);       \
+        } else if (tlist != nullptr) {                    \
+            if (bins) QSDVR_FWD(NC, true, true, false);   \
+            else QSDVR_FWD(NC, false, true, false);       \
+        } else {                                          \
+            if (bins) QSDVR_FWD(NC, true, false, false);  \
+            else QSDVR_FWD(NC, false, false, false);      \
+        }                                                 \
     } while (0)
     if (channels == 32) QSDVR_FWD_NC(9);
     else QSDVR_FWD_NC(16);

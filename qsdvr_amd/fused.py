@@ -23,11 +23,21 @@ SCATTER_TILE = 0     # tile-reduced atomics
 SCATTER_ATOMIC = 1   # one atomic per sample and channel
 SCATTER_DET = 2      # sorted-segment sums, bitwise reproducible
 
-# Whole-frame tile-mode renders record each 8x8 tile's expected backward
-# cost in the forward, and the adjoint then runs the tiles heaviest first
+# Whole-frame tile-mode renders record each 8x8 tile's expected backward cost
+# in the forward, and the adjoint then runs the tiles heaviest first
 # (qsdvr_fused_forward_costed / qsdvr_fused_backward_ordered): a fixed launch
-# order leaves the heaviest tiles to finish last.  False: the launch order.
+# order leaves the heaviest tiles to finish last.  TILE_ORDER False: the
+# launch order; TILE_ORDER_CHANNELS: the appearance channel counts it is
+# used for.  Not 53: the costed 53-channel forward compiles to a schedule
+# that measured 3.3 ms slower on a 384^3 grid at 800x800 with 48 listed
+# samples per ray, more than the ordering saves in its adjoint (0.7 ms).
 TILE_ORDER = True
+TILE_ORDER_CHANNELS = (32,)
+
+
+def _ordered(ray_pixel, scatter_mode, channels):
+    return (TILE_ORDER and ray_pixel is None and scatter_mode == SCATTER_TILE
+            and channels in TILE_ORDER_CHANNELS)
 
 # When set to a list, every fused kernel launch appends
 # (name, start_event, end_event) recorded on the launching stream.
@@ -96,7 +106,7 @@ class FusedRender(autograd.Function):
         depth = torch.empty((n_rays,), dtype=torch.float32, device=dev)
         cost = None
         with _timed("fused_forward"):
-            if TILE_ORDER and ray_pixel is None and scatter_mode == SCATTER_TILE:
+            if _ordered(ray_pixel, scatter_mode, Cc):
                 cost = torch.empty(_tile_count(n_cams, width, height), dtype=torch.int32,
                                    device=dev)
                 _lib.call("qsdvr_fused_forward_costed", _ptr(cam_buf), n_cams, width, height,

@@ -24,6 +24,7 @@ import torch
 from torch import autograd
 
 from . import _lib
+from . import fused as _fused
 from .fused import SCATTER_DET, SCATTER_TILE, backward_det, camera_buffer, _timed
 from .qsdvr import _ptr, _stream, _t
 
@@ -223,19 +224,30 @@ class ListedRender(autograd.Function):
         dev = sdf_grid.device
         color = torch.empty((n, 3), dtype=torch.float32, device=dev)
         depth = torch.empty((n,), dtype=torch.float32, device=dev)
+        cost = None
         with _timed("listed_forward"):
-            _lib.call("qsdvr_fused_forward_listed", _ptr(cam_buf), n_cams, W, H, _ptr(ray_pixel),
-                      n, _ptr(rs), _ptr(tlist), K, _ptr(sdf_grid), _ptr(data_grid),
-                      sdf_grid.shape[0], data_grid.shape[-1], float(interval), float(s),
-                      _ptr(color), _ptr(depth), _stream())
-        ctx.save_for_backward(sdf_grid, data_grid, cam_buf, ray_pixel, rs, tlist, color, depth)
+            if _fused._ordered(ray_pixel, scatter_mode, data_grid.shape[-1]):
+                # whole frames: record the tiles' costs for the ordered adjoint
+                cost = torch.empty(_fused._tile_count(n_cams, W, H), dtype=torch.int32,
+                                   device=dev)
+                _lib.call("qsdvr_fused_forward_costed", _ptr(cam_buf), n_cams, W, H, _ptr(rs),
+                          _ptr(tlist), K, _ptr(sdf_grid), _ptr(data_grid), sdf_grid.shape[0],
+                          data_grid.shape[-1], float(interval), float(s), _ptr(color),
+                          _ptr(depth), _ptr(cost), _stream())
+            else:
+                _lib.call("qsdvr_fused_forward_listed", _ptr(cam_buf), n_cams, W, H,
+                          _ptr(ray_pixel), n, _ptr(rs), _ptr(tlist), K, _ptr(sdf_grid),
+                          _ptr(data_grid), sdf_grid.shape[0], data_grid.shape[-1],
+                          float(interval), float(s), _ptr(color), _ptr(depth), _stream())
+        ctx.save_for_backward(sdf_grid, data_grid, cam_buf, ray_pixel, rs, tlist, color, depth,
+                              cost)
         ctx.args = (n_cams, W, H, n, K, float(interval), float(s), int(scatter_mode))
         ctx.set_materialize_grads(False)
         return color, depth
 
     @staticmethod
     def backward(ctx: Any, gcolor, gdepth):
-        sdf_grid, data_grid, cam_buf, ray_pixel, rs, tlist, color, depth = ctx.saved_tensors
+        sdf_grid, data_grid, cam_buf, ray_pixel, rs, tlist, color, depth, cost = ctx.saved_tensors
         n_cams, W, H, n, K, interval, s, mode = ctx.args
         none = (None,) * 10
         if gcolor is None and gdepth is None:
@@ -254,11 +266,19 @@ class ListedRender(autograd.Function):
                              interval, s, color, depth, gcolor.contiguous(), gdep, gq, gd)
             return (gq, gd) + none
         with _timed("listed_backward"):
-            _lib.call("qsdvr_fused_backward_listed", _ptr(cam_buf), n_cams, W, H,
-                      _ptr(ray_pixel), n, _ptr(rs), _ptr(tlist), K, _ptr(sdf_grid),
-                      _ptr(data_grid), sdf_grid.shape[0], data_grid.shape[-1], interval, s,
-                      _ptr(color), _ptr(depth), _ptr(gcolor.contiguous()), _ptr(gdep),
-                      _ptr(gq), _ptr(gd), mode, _stream())
+            if cost is not None:
+                ws, nb = _fused._order_workspace(n_cams, W, H, sdf_grid.device)
+                _lib.call("qsdvr_fused_backward_ordered", _ptr(cam_buf), n_cams, W, H, _ptr(rs),
+                          _ptr(tlist), K, _ptr(sdf_grid), _ptr(data_grid), sdf_grid.shape[0],
+                          data_grid.shape[-1], interval, s, _ptr(color), _ptr(depth),
+                          _ptr(gcolor.contiguous()), _ptr(gdep), _ptr(gq), _ptr(gd), _ptr(cost),
+                          _ptr(ws), nb, _stream())
+            else:
+                _lib.call("qsdvr_fused_backward_listed", _ptr(cam_buf), n_cams, W, H,
+                          _ptr(ray_pixel), n, _ptr(rs), _ptr(tlist), K, _ptr(sdf_grid),
+                          _ptr(data_grid), sdf_grid.shape[0], data_grid.shape[-1], interval, s,
+                          _ptr(color), _ptr(depth), _ptr(gcolor.contiguous()), _ptr(gdep),
+                          _ptr(gq), _ptr(gd), mode, _stream())
         return (gq, gd) + none
 
 

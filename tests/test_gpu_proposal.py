@@ -141,6 +141,38 @@ def test_listed_shortest_lists(mode, K):
     _listed_case(mode, 2, K)
 
 
+@pytest.mark.parametrize("sh_degree", [2, 3])
+def test_listed_cost_ordered_matches_launch_order(monkeypatch, sh_degree):
+    """Whole-frame listed renders in tile mode run their adjoint heaviest
+    tile first (the costed forward and the ordered backward): the same
+    colours and depths bit for bit as the launch order, and gradients to
+    1e-4 (float atomics in another order), on two poses in one call, at 32
+    and 53 appearance channels (53 is not ordered by default)."""
+    w = scenes.Workload("t", 24, sh_degree, 96, 80, 64, 3, "torus", 0.005)
+    grid, data, gen = scenes.build_grids(w, seed=3)
+    cams = scenes.orbit_cameras(3, w.width, w.height)[:2]
+    buf, rs = _state(cams, w.interval)
+    n = rs.shape[0]
+    z = torch.randn((n, 64), generator=torch.Generator().manual_seed(9))
+    t = proposal.sample(z.cuda(), rs, w.interval, K=24)
+    target = torch.rand((n, 3), generator=gen).cuda()
+    monkeypatch.setattr(fused, "TILE_ORDER_CHANNELS", (32, 53))
+    out = {}
+    for order in (False, True):
+        monkeypatch.setattr(fused, "TILE_ORDER", order)
+        assert fused._ordered(None, fused.SCATTER_TILE, data.shape[-1]) == order
+        G = grid.cuda().requires_grad_(True)
+        D = data.cuda().requires_grad_(True)
+        c, d = proposal.ListedRender.apply(G, D, buf, 2, w.width, w.height, None, rs, t,
+                                           w.interval, 8.0, fused.SCATTER_TILE)
+        (((c - target) ** 2).mean() + 0.01 * d.mean()).backward()
+        out[order] = (c.detach(), d.detach(), G.grad, D.grad)
+    assert torch.equal(out[False][0], out[True][0]) and torch.equal(out[False][1], out[True][1])
+    assert_close(out[True][2].reshape(-1, 7), out[False][2].reshape(-1, 7), 1e-4,
+                 "ordered vs launch-order listed sdf grad")
+    assert_close(out[True][3], out[False][3], 1e-4, "ordered vs launch-order listed appearance grad")
+
+
 def test_listed_uniform_is_plain_and_bins():
     w = scenes.Workload("t", 32, 2, 40, 30, 80, 2, "torus", 0.005)
     grid, data, _ = scenes.build_grids(w, seed=2)
