@@ -1474,6 +1474,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                 vofs = vx + vy * R + vz * (int)RR;
             }
             const int64_t vbase = (int64_t)lx + (int64_t)ly * R + (int64_t)lz * RR;
+            float *const vrow = gdata + vbase * C;  // the box's first vertex row
             wave_lds_order();
             // Appearance: box[v][ch] = sum over lanes l of W[l][v] * G[l][ch],
             // a (box vertices x 64) x (64 x C) product on the matrix cores
@@ -1561,13 +1562,18 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                             for (int i = 0; i < 4; ++i) vo[i] = __shfl(vofs, 16 * mb + 4 * g + i, kWave);
 #pragma unroll
                             for (int i = 0; i < 4; ++i) {
+                                // one branch per touched vertex, its channel
+                                // blocks' atomics addressed from one 32-bit offset
                                 const int v = 16 * mb + 4 * g + i;
+                                bool touched = false;
 #pragma unroll
-                                for (int q = 0; q < NBG; ++q) {
-                                    const int ch = 16 * (nb0 + q) + m;
-                                    const float d = acc[q][mb][i];
-                                    if (v < nv && ch < C && d != 0.0f)
-                                        atomic_addf(gdata + (vbase + vo[i]) * C + ch, d);
+                                for (int q = 0; q < NBG; ++q) touched |= acc[q][mb][i] != 0.0f;
+                                if (v < nv && touched) {
+                                    const uint32_t o = (uint32_t)(vo[i] * C + 16 * nb0 + m);
+#pragma unroll
+                                    for (int q = 0; q < NBG; ++q)
+                                        if (16 * (nb0 + q) + m < C)
+                                            atomic_addf(vrow + o + 16 * q, acc[q][mb][i]);
                                 }
                             }
                         }
