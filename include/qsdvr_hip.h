@@ -283,6 +283,30 @@ int qsdvr_sample_normal_backward_presorted(const float *grad_out, const float *s
                                            float *grad_grid, int64_t n, int32_t reso,
                                            void *workspace, size_t workspace_bytes,
                                            qsdvr_stream stream);
+/* GridInterpolationForward (kernel/render.cu:317-323, bound at
+ * kernel/qsdvr.cpp:8, prototype include/render.h:4) that first makes GridInterpolationBackward's sort of
+ * its samples (the sort qsdvr_grid_interpolation_backward_sorted makes) in
+ * `workspace` (qsdvr_sorted_scatter_workspace_size(n, (vertex_reso - 1)^3)
+ * bytes), then interpolates one run of a cell's samples at a time in that
+ * order.  Outputs equal qsdvr_grid_interpolation_forward's bit for bit (NaN
+ * rows for invalid indices).  32 channels, an even cell resolution, 16-B
+ * aligned data_grid and out. */
+int qsdvr_grid_interpolation_forward_sorted(const float *data_grid, const float *point,
+                                            const int32_t *index, float *out, int64_t n,
+                                            int32_t vertex_reso, int32_t channels,
+                                            void *workspace, size_t workspace_bytes,
+                                            qsdvr_stream stream);
+/* GridInterpolationBackward (kernel/render.cu:325-331, bound at
+ * kernel/qsdvr.cpp:10, prototype include/render.h:6) on the sort that a preceding
+ * qsdvr_grid_interpolation_forward_sorted call made of the same point and
+ * index lists: `workspace` is that call's, unchanged since (the points and
+ * indices come from there, so the reference's pointList and indexList are
+ * not taken).  Same sums as qsdvr_grid_interpolation_backward_sorted; the
+ * workspace then also serves qsdvr_sample_normal_backward_presorted. */
+int qsdvr_grid_interpolation_backward_presorted(float *grad_grid, const float *grad_out,
+                                                int64_t n, int32_t vertex_reso,
+                                                int32_t channels, void *workspace,
+                                                size_t workspace_bytes, qsdvr_stream stream);
 
 /* ------------------------------------------------------------- fused --- */
 

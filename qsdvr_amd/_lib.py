@@ -66,6 +66,8 @@ SIGNATURES = {
     "qsdvr_sample_sdf_backward_sorted": ([P, P, P, P, P, I64, I64, F, P, SZ, P], C.c_int),
     "qsdvr_sample_normal_backward_sorted": ([P, P, P, P, P, I64, I64, P, SZ, P], C.c_int),
     "qsdvr_sample_normal_backward_presorted": ([P, P, P, I64, I32, P, SZ, P], C.c_int),
+    "qsdvr_grid_interpolation_forward_sorted": ([P, P, P, P, I64, I32, I32, P, SZ, P], C.c_int),
+    "qsdvr_grid_interpolation_backward_presorted": ([P, P, I64, I32, I32, P, SZ, P], C.c_int),
     "qsdvr_fused_forward": ([P, I32, I32, I32, P, I64, P, P, I32, I32, F, F, P, P, P],
                             C.c_int),
     "qsdvr_fused_backward": ([P, I32, I32, I32, P, I64, P, P, I32, I32, F, F, P, P, P, P, P,

@@ -733,6 +733,117 @@ __global__ void __launch_bounds__(256) op_app_reduce32_kernel(
     }
 }
 
+// GridInterpolationForward (kernel/render.cu:317-323, the lerp order of
+// models/render_torch.py:128-143) over the samples in GridInterpolation's
+// sorted order, C = 32: a wavefront takes one run of a cell's samples at a
+// time with lane = (channel quad q, sample slot b).  Lane (q, b) loads quad q
+// of the cell's vertex row b -- one 16-B load per lane brings all 8 rows --
+// and ds_swizzle broadcasts give each lane quad q of the 8 rows; then every
+// 8 samples cost one 16-B load of their (point, index) records and one
+// 16-B store per lane, against 8 row loads per 8 samples when each sample
+// gathers its own rows (the address unit's cost is per instruction).  The
+// outputs are the per-sample gather's, bit for bit.  Waves take contiguous
+// ranges of sorted positions (neighbouring cells share rows in one XCD's
+// L2), request each chunk's keys one chunk ahead and find run heads with a
+// lane shuffle.  Samples with an invalid index (sorted last) get NaN.
+template <int BCAST>
+__device__ __forceinline__ float4 bcast8(float4 v) {
+    // bit-mode swizzle within 32 lanes: lane (l & 0x18) | BCAST
+    constexpr int pat = 0x18 | (BCAST << 5);
+    return float4{__int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v.x), pat)),
+                  __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v.y), pat)),
+                  __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v.z), pat)),
+                  __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v.w), pat))};
+}
+
+__device__ __forceinline__ float4 lerp4s(float4 a, float4 b, float w) {
+    return float4{lerp_ref(a.x, b.x, w), lerp_ref(a.y, b.y, w), lerp_ref(a.z, b.z, w),
+                  lerp_ref(a.w, b.w, w)};
+}
+
+__global__ void __launch_bounds__(256) op_interp_fwd_runs32_kernel(
+    const uint32_t *__restrict__ key, const PointRec *__restrict__ pay,
+    const float4 *__restrict__ data, const uint32_t *__restrict__ bounds, ClassKeys ck,
+    int64_t n, int64_t chunks_per_wave, float4 *__restrict__ out) {
+    const int64_t hi = bounds[8];  // valid samples: sorted positions [0, hi)
+    const int lane = threadIdx.x % kWave;
+    const int q = lane >> 3, b = lane & 7;
+    const int64_t R = ck.R, RR = R * R;
+    const int64_t wave = xcd_block() * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int64_t c0 = wave * chunks_per_wave;
+    const int64_t pend = std::min<int64_t>((c0 + chunks_per_wave) * kWave, hi);
+    auto key_at = [&](int64_t p) { return p < hi ? key[p] : 0xFFFFFFFFu; };
+    int64_t p0 = c0 * kWave;
+    // the key before the range, and this chunk's keys; each chunk's keys
+    // are requested one chunk ahead
+    uint32_t prev = p0 > 0 && p0 < pend ? key[p0 - 1] : 0xFFFFFFFFu;
+    uint32_t k = p0 < pend ? key_at(p0 + lane) : 0xFFFFFFFFu;
+    for (; p0 < pend; p0 += kWave) {
+        // (also past the range's end: its last run may continue into the next range)
+        const uint32_t kn = p0 + kWave < hi ? key_at(p0 + kWave + lane) : 0xFFFFFFFFu;
+        const int64_t p = p0 + lane;
+        uint32_t left = __shfl_up(k, 1, kWave);
+        if (lane == 0) left = prev;
+        prev = __shfl(k, kWave - 1, kWave);
+        const bool boundary = p < hi && (p == 0 || left != k);
+        const unsigned long long bmask = __ballot(boundary);
+        if (bmask != 0) {
+            const int last = 63 - __clzll((long long)bmask);
+            // the last run's end: in the next chunk's keys, else further on
+            const uint32_t lk = __shfl(k, last, kWave);
+            const unsigned long long nx = __ballot(kn != lk);
+            const int64_t tail_end = nx != 0 ? std::min<int64_t>(p0 + kWave + (__ffsll((long long)nx) - 1), hi)
+                                             : run_end(key, p0 + 2 * kWave, hi, lk);
+            unsigned long long heads = bmask;
+            while (heads != 0) {
+                const int h = __ffsll((long long)heads) - 1;
+                heads &= heads - 1;
+                const int64_t start = p0 + h;
+                const int64_t end = heads != 0 ? p0 + (__ffsll((long long)heads) - 1) : tail_end;
+                const int64_t cell = ck.cell(__shfl(k, h, kWave));
+                const int64_t row = cell + (b & 1) + ((b >> 1) & 1) * R + (b >> 2) * RR;
+                // 16 samples a round trip (slot b takes i0 + b and i0 + 8 + b),
+                // the next round's records requested before this round's
+                // stores; the first round's records issued with the row quad
+                auto rec_at = [&](int64_t i) {
+                    return *reinterpret_cast<const float4 *>(pay + (i < end ? i : start));
+                };
+                float4 na = rec_at(start + b), nb = rec_at(start + 8 + b);
+                const float4 mine = data[row * 8 + q];
+                __builtin_amdgcn_sched_barrier(0);
+                const float4 v0 = bcast8<0>(mine), v1 = bcast8<1>(mine), v2 = bcast8<2>(mine),
+                             v3 = bcast8<3>(mine), v4 = bcast8<4>(mine), v5 = bcast8<5>(mine),
+                             v6 = bcast8<6>(mine), v7 = bcast8<7>(mine);
+                auto interp = [&](float4 r) {
+                    const float4 a00 = lerp4s(v0, v1, r.x), a01 = lerp4s(v2, v3, r.x);
+                    const float4 a0 = lerp4s(a00, a01, r.y);
+                    const float4 a10 = lerp4s(v4, v5, r.x), a11 = lerp4s(v6, v7, r.x);
+                    const float4 a1 = lerp4s(a10, a11, r.y);
+                    return lerp4s(a0, a1, r.z);
+                };
+                for (int64_t i0 = start; i0 < end; i0 += 16) {
+                    const float4 ra = na, rb = nb;
+                    if (i0 + 16 < end) {
+                        na = rec_at(i0 + 16 + b);
+                        nb = rec_at(i0 + 24 + b);
+                    }
+                    const float4 oa = interp(ra), ob = interp(rb);
+                    if (i0 + b < end) out[(int64_t)__float_as_uint(ra.w) * 8 + q] = oa;
+                    if (i0 + 8 + b < end) out[(int64_t)__float_as_uint(rb.w) * 8 + q] = ob;
+                }
+            }
+        }
+        k = kn;
+    }
+    // the invalid samples (normally none): NaN rows, 8 samples a wave step
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    const float nan = __builtin_nanf("");
+    for (int64_t i = hi + wave * 8 + b; i < n; i += nwaves * 8) {
+        const uint32_t idx = pay[i].i;
+        out[(int64_t)idx * 8 + q] = float4{nan, nan, nan, nan};
+    }
+}
+
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 int det_reduce(DetBuffers &b, int64_t T, float *gq, float *gdata, hipStream_t s) {
@@ -880,6 +991,52 @@ int op_prepare(void *ws, size_t bytes, int64_t n, int64_t cells, OpBuffers &b, c
     return QSDVR_OK;
 }
 
+// GridInterpolation's sort: ClassKeys keys of the samples, sorted with
+// their (point, index) records, and bounds[0..8] of the parity classes in
+// the sorted keys (bounds[8] starts the invalid samples).
+int interp_sort(const float *point, const int32_t *index, int64_t n, int R, OpBuffers &b,
+                hipStream_t s) {
+    const ClassKeys ck = class_keys(R);
+    PointRec *pay = reinterpret_cast<PointRec *>(b.val), *pay_s = reinterpret_cast<PointRec *>(b.val_s);
+    op_interp_keys_kernel<<<grid_1d(n, 256), 256, 0, s>>>(index, point, n, ck, b.key, pay);
+    QSDVR_LAUNCHED();
+    size_t tb = b.temp_bytes;
+    QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.key, b.key_s, pay, pay_s, (size_t)n, 0u,
+                                        (unsigned)bits_above((uint64_t)R * R * R), s));
+    det_class_bounds_kernel<<<1, 64, 0, s>>>(b.key_s, n, ck.h3, b.bounds);
+    QSDVR_LAUNCHED();
+    return QSDVR_OK;
+}
+
+// GridInterpolationBackward's sums on that sort: 8 passes, one per cell
+// parity class (cells of one class share no vertex row).
+int interp_class_sums(float *grad_grid, const float *grad_out, int64_t n, int R, int channels,
+                      const OpBuffers &b, hipStream_t s) {
+    const ClassKeys ck = class_keys(R);
+    const PointRec *pay_s = reinterpret_cast<const PointRec *>(b.val_s);
+    const int64_t chunks = (n / 8 + kWave - 1) / kWave;
+    const unsigned blocks =
+        (unsigned)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 4096));
+    const bool quads = channels == 32 && aligned16(grad_out) && aligned16(grad_grid);
+    for (int cls = 0; cls < 8; ++cls) {
+        // (one run per wavefront or per half: a frame's samples sorted by cell
+        // make long, uneven runs, and a wavefront of 8 runs waits for its
+        // longest)
+        if (quads)
+            op_app_reduce32_kernel<<<blocks, 256, 0, s>>>(
+                b.key_s, pay_s, reinterpret_cast<const float4 *>(grad_out), b.bounds, cls, ck,
+                grad_grid);
+        else if (channels == 32)
+            op_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(b.key_s, pay_s, grad_out, b.bounds,
+                                                           cls, ck, grad_grid);
+        else
+            op_app_reduce_kernel<53><<<blocks, 256, 0, s>>>(b.key_s, pay_s, grad_out, b.bounds,
+                                                           cls, ck, grad_grid);
+        QSDVR_LAUNCHED();
+    }
+    return QSDVR_OK;
+}
+
 }  // namespace
 
 }  // namespace qsdvr
@@ -907,42 +1064,68 @@ int qsdvr_grid_interpolation_backward_sorted(float *grad_grid, const float *poin
     QSDVR_REQUIRE(grad_grid && point && index && grad_out,
                   "qsdvr_grid_interpolation_backward_sorted: null pointer");
     const int R = vertex_reso - 1;
-    const int64_t cells = (int64_t)R * R * R;
     OpBuffers b;
-    int st = op_prepare(workspace, workspace_bytes, n, cells,
-                        b, "qsdvr_grid_interpolation_backward_sorted");
+    int st = op_prepare(workspace, workspace_bytes, n, (int64_t)R * R * R, b,
+                        "qsdvr_grid_interpolation_backward_sorted");
     if (st != QSDVR_OK) return st;
     hipStream_t s = as_stream(stream);
-    const ClassKeys ck = class_keys(R);
-    PointRec *pay = reinterpret_cast<PointRec *>(b.val), *pay_s = reinterpret_cast<PointRec *>(b.val_s);
-    op_interp_keys_kernel<<<grid_1d(n, 256), 256, 0, s>>>(index, point, n, ck, b.key, pay);
+    st = interp_sort(point, index, n, R, b, s);
+    if (st != QSDVR_OK) return st;
+    return interp_class_sums(grad_grid, grad_out, n, R, channels, b, s);
+}
+
+int qsdvr_grid_interpolation_forward_sorted(const float *data_grid, const float *point,
+                                            const int32_t *index, float *out, int64_t n,
+                                            int32_t vertex_reso, int32_t channels,
+                                            void *workspace, size_t workspace_bytes,
+                                            qsdvr_stream stream) {
+    QSDVR_REQUIRE(n >= 0 && vertex_reso >= 3 && (vertex_reso - 1) % 2 == 0 && channels == 32,
+                  "qsdvr_grid_interpolation_forward_sorted: bad sizes (even cell reso, "
+                  "channels 32)");
+    if (n == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(data_grid && point && index && out,
+                  "qsdvr_grid_interpolation_forward_sorted: null pointer");
+    QSDVR_REQUIRE(aligned16(data_grid) && aligned16(out),
+                  "qsdvr_grid_interpolation_forward_sorted: data_grid and out must be 16-B "
+                  "aligned");
+    const int R = vertex_reso - 1;
+    OpBuffers b;
+    int st = op_prepare(workspace, workspace_bytes, n, (int64_t)R * R * R, b,
+                        "qsdvr_grid_interpolation_forward_sorted");
+    if (st != QSDVR_OK) return st;
+    hipStream_t s = as_stream(stream);
+    st = interp_sort(point, index, n, R, b, s);
+    if (st != QSDVR_OK) return st;
+    // 4 waves a block; each wave a contiguous range of 64-position chunks
+    const int64_t chunks = (n + kWave - 1) / kWave;
+    const int64_t waves = std::max<int64_t>(1, std::min<int64_t>(chunks, 65536));
+    const int64_t per_wave = (chunks + waves - 1) / waves;
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    op_interp_fwd_runs32_kernel<<<blocks, 256, 0, s>>>(
+        b.key_s, reinterpret_cast<const PointRec *>(b.val_s),
+        reinterpret_cast<const float4 *>(data_grid), b.bounds, class_keys(R), n, per_wave,
+        reinterpret_cast<float4 *>(out));
     QSDVR_LAUNCHED();
-    size_t tb = b.temp_bytes;
-    QSDVR_HIP(rocprim::radix_sort_pairs(b.temp, tb, b.key, b.key_s, pay, pay_s, (size_t)n, 0u,
-                                        (unsigned)bits_above((uint64_t)cells), s));
-    det_class_bounds_kernel<<<1, 64, 0, s>>>(b.key_s, n, ck.h3, b.bounds);
-    QSDVR_LAUNCHED();
-    const int64_t chunks = (n / 8 + kWave - 1) / kWave;
-    const unsigned blocks =
-        (unsigned)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 4096));
-    const bool quads = channels == 32 && aligned16(grad_out) && aligned16(grad_grid);
-    for (int cls = 0; cls < 8; ++cls) {
-        // (one run per wavefront or per half: a frame's samples sorted by cell
-        // make long, uneven runs, and a wavefront of 8 runs waits for its
-        // longest)
-        if (quads)
-            op_app_reduce32_kernel<<<blocks, 256, 0, s>>>(
-                b.key_s, pay_s, reinterpret_cast<const float4 *>(grad_out), b.bounds, cls, ck,
-                grad_grid);
-        else if (channels == 32)
-            op_app_reduce_kernel<32><<<blocks, 256, 0, s>>>(b.key_s, pay_s, grad_out, b.bounds,
-                                                           cls, ck, grad_grid);
-        else
-            op_app_reduce_kernel<53><<<blocks, 256, 0, s>>>(b.key_s, pay_s, grad_out, b.bounds,
-                                                           cls, ck, grad_grid);
-        QSDVR_LAUNCHED();
-    }
     return QSDVR_OK;
+}
+
+int qsdvr_grid_interpolation_backward_presorted(float *grad_grid, const float *grad_out,
+                                                int64_t n, int32_t vertex_reso,
+                                                int32_t channels, void *workspace,
+                                                size_t workspace_bytes, qsdvr_stream stream) {
+    QSDVR_REQUIRE(n >= 0 && vertex_reso >= 3 && (vertex_reso - 1) % 2 == 0 &&
+                      (channels == 32 || channels == 53),
+                  "qsdvr_grid_interpolation_backward_presorted: bad sizes (even cell reso, "
+                  "channels 32 or 53)");
+    if (n == 0) return QSDVR_OK;
+    QSDVR_REQUIRE(grad_grid && grad_out,
+                  "qsdvr_grid_interpolation_backward_presorted: null pointer");
+    const int R = vertex_reso - 1;
+    OpBuffers b;
+    int st = op_prepare(workspace, workspace_bytes, n, (int64_t)R * R * R, b,
+                        "qsdvr_grid_interpolation_backward_presorted");
+    if (st != QSDVR_OK) return st;
+    return interp_class_sums(grad_grid, grad_out, n, R, channels, b, as_stream(stream));
 }
 
 int qsdvr_sample_sdf_backward_sorted(const float *grad_out, const float *sdf_grid,

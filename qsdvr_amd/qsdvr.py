@@ -143,11 +143,13 @@ def _op(fn):
 
     @functools.wraps(fn)
     def wrapped(*args, **kwargs):
-        global _calls, _render_sort
+        global _calls, _render_sort, _forward_sort
         _calls += 1
         result = fn(*args, **kwargs)
         if _render_sort is not None and _render_sort[-1] != _calls:
             _render_sort = None  # an earlier call's sort: no later call may use it
+        if _forward_sort is not None and _forward_sort[0]() is None:
+            _forward_sort = None  # its samples are gone: free the workspace
         return result
     return wrapped
 
@@ -165,6 +167,13 @@ def GridInterpolationForward(dataGrid, pointList, indexList, out, reso):
     R1 = int(reso)
     if dataGrid.numel() != R1 ** 3 * Cc:
         raise ValueError(f"dataGrid has {dataGrid.numel()} elements, expected ({R1})^3 x {Cc}")
+    if interp_forward_path(n, R1, Cc, dataGrid, out) == "sorted":
+        ws_n = _sorted_workspace(n, (R1 - 1) ** 3)
+        ws = _workspace(ws_n)
+        _lib.call("qsdvr_grid_interpolation_forward_sorted", _ptr(dataGrid), _ptr(pointList),
+                  _ptr(indexList), _ptr(out), n, R1, Cc, _ptr(ws), ws_n, _stream())
+        _remember_forward_sort(pointList, indexList, n, R1 - 1, ws, ws_n)
+        return
     _lib.call("qsdvr_grid_interpolation_forward", _ptr(dataGrid), _ptr(pointList),
               _ptr(indexList), _ptr(out), n, R1, Cc, _stream())
 
@@ -182,6 +191,13 @@ def GridInterpolationBackward(dataGrid, pointList, indexList, out, reso):
     if dataGrid.numel() != R1 ** 3 * Cc:
         raise ValueError(f"dataGrid has {dataGrid.numel()} elements, expected ({R1})^3 x {Cc}")
     if interp_backward_path(n, R1, Cc) == "sorted":
+        fwd = _take_forward_sort(pointList, indexList, n, R1 - 1)
+        if fwd is not None:  # GridInterpolationForward sorted these samples
+            ws, ws_n = fwd
+            _lib.call("qsdvr_grid_interpolation_backward_presorted", _ptr(dataGrid), _ptr(out),
+                      n, R1, Cc, _ptr(ws), ws_n, _stream())
+            _remember_render_sort(pointList, indexList, n, R1 - 1, ws, ws_n)
+            return
         ws_n = _sorted_workspace(n, (R1 - 1) ** 3)
         ws = _workspace(ws_n)
         _lib.call("qsdvr_grid_interpolation_backward_sorted", _ptr(dataGrid), _ptr(pointList),
@@ -383,6 +399,17 @@ def interp_backward_path(n, vertex_reso, channels):
     return "sorted" if ok else "atomic"
 
 
+def interp_forward_path(n, vertex_reso, channels, dataGrid=None, out=None):
+    """"sorted" or "gather": which GridInterpolationForward kernel runs.  The
+    sorted form makes GridInterpolationBackward's sort of the samples (so the
+    backward on the same lists skips it) and interpolates one cell's run of
+    samples at a time: the sorted backward's sizes, 32 channels, and 16-B
+    aligned grid and output rows."""
+    aligned = all(t is None or t.data_ptr() % 16 == 0 for t in (dataGrid, out))
+    ok = interp_backward_path(n, vertex_reso, channels) == "sorted" and channels == 32 and aligned
+    return "sorted" if ok else "gather"
+
+
 def quadric_backward_path(n, sdfGrid):
     """"sorted" or "atomic": which SampleSDF / SampleNormal backward kernel
     runs for ``SCATTER``."""
@@ -428,6 +455,39 @@ def _take_render_sort(points, index, n, cells):
             or R ** 3 != int(cells) or stream != torch.cuda.current_stream().cuda_stream):
         return None
     return R, ws, ws_n
+
+
+# GridInterpolationForward's sort, for the GridInterpolationBackward of the
+# same samples (models/render.py saves the point and index lists for it):
+# one entry, with weak references to the point and index tensors and their
+# version counters (an in-place change invalidates it), the size and
+# resolution and the stream.  The matching backward takes it; the next
+# sorted forward replaces it; it is dropped when its point list is freed.
+# Between forward and backward it holds the workspace (about 41 B per sample
+# plus the sort's temporary storage).
+_forward_sort = None
+
+
+def _remember_forward_sort(points, index, n, R, ws, ws_n):
+    global _forward_sort
+    import weakref
+    _forward_sort = (weakref.ref(points), points._version, weakref.ref(index), index._version,
+                     int(n), int(R), ws, int(ws_n), torch.cuda.current_stream().cuda_stream)
+
+
+def _take_forward_sort(points, index, n, R):
+    """(workspace, bytes) of a matching GridInterpolationForward sort, or None;
+    the entry is consumed either way."""
+    global _forward_sort
+    ent, _forward_sort = _forward_sort, None
+    if ent is None:
+        return None
+    pref, pversion, iref, iversion, n0, R0, ws, ws_n, stream = ent
+    if (pref() is not points or points._version != pversion or iref() is not index
+            or index._version != iversion or n0 != int(n) or R0 != int(R)
+            or stream != torch.cuda.current_stream().cuda_stream):
+        return None
+    return ws, ws_n
 
 
 def _sorted_workspace(n, cells):

@@ -119,3 +119,24 @@ def test_sorted_scatters_validate_on_the_host():
     st = lib.qsdvr_sample_normal_backward_presorted(None, dummy, dummy, 100, 8, dummy, 1 << 20,
                                                     None)
     assert st == -1 and b"null" in lib.qsdvr_last_error()
+    # the sorted GridInterpolationForward (32 channels, even resolution, 16-B
+    # aligned grid and output) and the backward on its sort
+    fwd = lib.qsdvr_grid_interpolation_forward_sorted
+    assert fwd(None, None, None, None, 0, 9, 32, None, 0, None) == 0
+    st = fwd(dummy, dummy, dummy, dummy, 100, 10, 32, dummy, 1 << 20, None)
+    assert st == -1 and b"even" in lib.qsdvr_last_error()
+    assert fwd(dummy, dummy, dummy, dummy, 100, 9, 53, dummy, 1 << 20, None) == -1
+    st = fwd(C.c_void_p(20), dummy, dummy, dummy, 100, 9, 32, dummy, 1 << 30, None)
+    assert st == -1 and b"aligned" in lib.qsdvr_last_error()
+    st = fwd(dummy, dummy, dummy, dummy, 100, 9, 32, dummy, 16, None)
+    assert st == -4 and b"too small" in lib.qsdvr_last_error()
+    st = fwd(dummy, None, dummy, dummy, 100, 9, 32, dummy, 1 << 30, None)
+    assert st == -1 and b"null" in lib.qsdvr_last_error()
+    pre = lib.qsdvr_grid_interpolation_backward_presorted
+    assert pre(None, None, 0, 9, 32, None, 0, None) == 0
+    st = pre(dummy, dummy, 100, 10, 32, dummy, 1 << 20, None)
+    assert st == -1 and b"even" in lib.qsdvr_last_error()
+    st = pre(dummy, dummy, 100, 9, 32, dummy, 16, None)
+    assert st == -4 and b"too small" in lib.qsdvr_last_error()
+    st = pre(dummy, None, 100, 9, 53, dummy, 1 << 30, None)
+    assert st == -1 and b"null" in lib.qsdvr_last_error()

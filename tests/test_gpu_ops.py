@@ -370,6 +370,130 @@ def test_normal_backward_reuses_the_interpolation_sort(monkeypatch):
     assert "qsdvr_sample_normal_backward_presorted" not in names
 
 
+def test_sorted_interpolation_forward_equals_the_gather(monkeypatch):
+    """GridInterpolationForward on its sort (one cell's run of samples at a
+    time) gives the per-sample gather's rows bit for bit: on a full 800x800
+    pose of the 128^3 torus, and on random samples with out-of-range indices
+    (NaN rows).  An output 4 B off 16-B alignment takes the gather."""
+    from qsdvr_amd import _lib
+    from qsdvr_amd.camera import Camera
+    from qsdvr_amd.qsdvr import _ptr, _stream
+    names = []
+    real_call = _lib.call
+    monkeypatch.setattr(_lib, "call", lambda name, *a: (names.append(name), real_call(name, *a)))
+    monkeypatch.setattr(qsdvr, "SCATTER", "sorted")
+    w = scenes.WORKLOADS["plain"]
+    cam = scenes.orbit_cameras(100, w.width, w.height)[31]
+    grid, data, _ = scenes.build_grids(w, seed=0, device="cuda")
+    cc = Camera(qsdvr.CameraArgs.from_camera(cam), w.interval, w.R)
+    cc.GenerateRayPoints(None)
+    inp = cc.GetRayPoints()
+    rp, ri = inp["renderPointList"], inp["renderIndexList"]
+    n = rp.shape[0]
+    out = torch.empty((n, 32), device="cuda")
+    qsdvr.GridInterpolationForward(data, rp, ri, out, w.R + 1)
+    assert names[-1] == "qsdvr_grid_interpolation_forward_sorted"
+    ref = torch.empty_like(out)
+    real_call("qsdvr_grid_interpolation_forward", _ptr(data), _ptr(rp), _ptr(ri), _ptr(ref), n,
+              w.R + 1, 32, _stream())
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    assert float(out.abs().sum()) > 0
+
+    R = 10
+    g = torch.Generator(device="cuda").manual_seed(12)
+    m = 30011
+    idx = torch.randint(-5, R ** 3 + 5, (m,), generator=g, device="cuda").to(torch.int32)
+    pts = torch.rand((m, 3), generator=g, device="cuda") * 2 - 1
+    dg = torch.randn(((R + 1) ** 3, 32), generator=g, device="cuda")
+    o1 = torch.empty((m, 32), device="cuda")
+    qsdvr.GridInterpolationForward(dg, pts, idx, o1, R + 1)
+    assert names[-1] == "qsdvr_grid_interpolation_forward_sorted"
+    o2 = torch.empty_like(o1)
+    real_call("qsdvr_grid_interpolation_forward", _ptr(dg), _ptr(pts), _ptr(idx), _ptr(o2), m,
+              R + 1, 32, _stream())
+    torch.cuda.synchronize()
+    bad = (idx < 0) | (idx >= R ** 3)
+    assert int(bad.sum()) > 0
+    assert bool(torch.isnan(o1[bad]).all()) and bool(torch.isfinite(o1[~bad]).all())
+    assert torch.equal(o1[~bad], o2[~bad])
+    o3 = torch.empty(m * 32 + 1, device="cuda")[1:].view(m, 32)
+    qsdvr.GridInterpolationForward(dg, pts, idx, o3, R + 1)
+    assert names[-1] == "qsdvr_grid_interpolation_forward"
+    assert torch.equal(o3[~bad], o2[~bad])
+
+
+def test_interpolation_backward_reuses_the_forward_sort(monkeypatch):
+    """On a full 800x800 pose: GridInterpolationBackward on the point and index
+    lists of the preceding sorted GridInterpolationForward sums on that
+    forward's sort (no keys, no sort) and gives its own sort's sums bit for
+    bit, even with other op calls in between; SampleNormalBackward then
+    reuses the same sort.  An in-place change of the indices, other index
+    tensors, or the point list being freed drops the kept sort."""
+    import gc
+    from qsdvr_amd import _lib
+    from qsdvr_amd.camera import Camera
+    names = []
+    real_call = _lib.call
+    monkeypatch.setattr(_lib, "call", lambda name, *a: (names.append(name), real_call(name, *a)))
+    monkeypatch.setattr(qsdvr, "SCATTER", "sorted")
+    w = scenes.WORKLOADS["plain"]
+    cam = scenes.orbit_cameras(100, w.width, w.height)[41]
+    grid, data, _ = scenes.build_grids(w, seed=0, device="cuda")
+    cc = Camera(qsdvr.CameraArgs.from_camera(cam), w.interval, w.R)
+    cc.GenerateRayPoints(None)
+    inp = cc.GetRayPoints()
+    rp, ri = inp["renderPointList"], inp["renderIndexList"]
+    n = rp.shape[0]
+    g = torch.Generator(device="cuda").manual_seed(4)
+    up_d = torch.randn((n, 32), generator=g, device="cuda")
+    up_n = torch.randn((n, 3), generator=g, device="cuda")
+
+    def forward():
+        qsdvr.GridInterpolationForward(data, rp, ri, torch.empty((n, 32), device="cuda"),
+                                       w.R + 1)
+
+    def backward(points=rp, index=ri):
+        gd = torch.zeros_like(data)
+        qsdvr.GridInterpolationBackward(gd, points, index, up_d, w.R + 1)
+        return gd
+
+    def normal():
+        gn = torch.zeros_like(grid)
+        qsdvr.SampleNormalBackward(up_n, grid, rp, ri, gn)
+        return gn
+
+    own = backward()
+    own_n = normal()
+    names.clear()
+    forward()
+    qsdvr.SampleNormalForward(torch.empty((n, 3), device="cuda"), grid, rp, ri)  # in between
+    reused = backward()
+    reused_n = normal()
+    torch.cuda.synchronize()
+    assert names.count("qsdvr_grid_interpolation_backward_presorted") == 1
+    assert "qsdvr_grid_interpolation_backward_sorted" not in names
+    assert names.count("qsdvr_sample_normal_backward_presorted") == 1
+    assert qsdvr._forward_sort is None
+    assert torch.equal(reused, own) and torch.equal(reused_n, own_n)
+    names.clear()
+    forward()
+    ri.add_(0)                                    # in-place: new version
+    assert torch.equal(backward(), own)
+    assert "qsdvr_grid_interpolation_backward_presorted" not in names
+    names.clear()
+    forward()
+    assert torch.equal(backward(index=ri.clone()), own)  # other index tensor
+    assert "qsdvr_grid_interpolation_backward_presorted" not in names
+    rp2, ri2 = rp.clone(), ri.clone()
+    qsdvr.GridInterpolationForward(data, rp2, ri2, torch.empty((n, 32), device="cuda"), w.R + 1)
+    assert qsdvr._forward_sort is not None
+    del rp2
+    gc.collect()
+    qsdvr.SampleNormalForward(torch.empty((n, 3), device="cuda"), grid, rp, ri)
+    assert qsdvr._forward_sort is None             # its point list freed: workspace dropped
+
+
 def test_sorted_scatters_accept_an_unaligned_workspace():
     """The C ABI's sorted scatters take a caller workspace of the queried size
     at any address: its slices are placed at 256-B aligned addresses inside
