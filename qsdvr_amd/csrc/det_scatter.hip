@@ -794,23 +794,38 @@ __global__ void __launch_bounds__(256) op_interp_fwd_runs32_kernel(
             const unsigned long long nx = __ballot(kn != lk);
             const int64_t tail_end = nx != 0 ? std::min<int64_t>(p0 + kWave + (__ffsll((long long)nx) - 1), hi)
                                              : run_end(key, p0 + 2 * kWave, hi, lk);
+            // The chunk's runs one after another, each run's row quad and first
+            // records requested while the run before it is interpolated.
+            struct RunIn {
+                int64_t start, end;
+                float4 mine, na, nb;
+            };
             unsigned long long heads = bmask;
-            while (heads != 0) {
+            auto fetch = [&]() {
+                RunIn r;
                 const int h = __ffsll((long long)heads) - 1;
                 heads &= heads - 1;
-                const int64_t start = p0 + h;
-                const int64_t end = heads != 0 ? p0 + (__ffsll((long long)heads) - 1) : tail_end;
+                r.start = p0 + h;
+                r.end = heads != 0 ? p0 + (__ffsll((long long)heads) - 1) : tail_end;
                 const int64_t cell = ck.cell(__shfl(k, h, kWave));
                 const int64_t row = cell + (b & 1) + ((b >> 1) & 1) * R + (b >> 2) * RR;
+                r.na = *reinterpret_cast<const float4 *>(pay + std::min(r.start + b, r.end - 1));
+                r.nb = *reinterpret_cast<const float4 *>(pay + std::min(r.start + 8 + b, r.end - 1));
+                r.mine = data[row * 8 + q];
+                return r;
+            };
+            RunIn cur = fetch();
+            for (;;) {
+                const bool more = heads != 0;
+                RunIn nxt = cur;
+                if (more) nxt = fetch();
+                const int64_t start = cur.start, end = cur.end;
                 // 16 samples a round trip (slot b takes i0 + b and i0 + 8 + b),
-                // the next round's records requested before this round's
-                // stores; the first round's records issued with the row quad
+                // the next round's records requested before this round's stores
                 auto rec_at = [&](int64_t i) {
                     return *reinterpret_cast<const float4 *>(pay + (i < end ? i : start));
                 };
-                float4 na = rec_at(start + b), nb = rec_at(start + 8 + b);
-                const float4 mine = data[row * 8 + q];
-                __builtin_amdgcn_sched_barrier(0);
+                const float4 mine = cur.mine;
                 const float4 v0 = bcast8<0>(mine), v1 = bcast8<1>(mine), v2 = bcast8<2>(mine),
                              v3 = bcast8<3>(mine), v4 = bcast8<4>(mine), v5 = bcast8<5>(mine),
                              v6 = bcast8<6>(mine), v7 = bcast8<7>(mine);
@@ -821,6 +836,7 @@ __global__ void __launch_bounds__(256) op_interp_fwd_runs32_kernel(
                     const float4 a1 = lerp4s(a10, a11, r.y);
                     return lerp4s(a0, a1, r.z);
                 };
+                float4 na = cur.na, nb = cur.nb;
                 for (int64_t i0 = start; i0 < end; i0 += 16) {
                     const float4 ra = na, rb = nb;
                     if (i0 + 16 < end) {
@@ -831,6 +847,8 @@ __global__ void __launch_bounds__(256) op_interp_fwd_runs32_kernel(
                     if (i0 + b < end) out[(int64_t)__float_as_uint(ra.w) * 8 + q] = oa;
                     if (i0 + 8 + b < end) out[(int64_t)__float_as_uint(rb.w) * 8 + q] = ob;
                 }
+                if (!more) break;
+                cur = nxt;
             }
         }
         k = kn;
