@@ -143,3 +143,14 @@ def call(name: str, *args) -> None:
 
 def fn(name: str):
     return getattr(load(), name)
+
+
+# The library is loaded when this module is first imported (by the op, fused,
+# proposal and training modules), ahead of the caller's first GPU
+# allocations: in the first process on a GPU, loading it only after the
+# grids were allocated measured the fused adjoint 15-35% slower for the
+# whole process.  A missing library is reported at the first op call.
+try:
+    load()
+except (RuntimeError, OSError):
+    pass
