@@ -109,6 +109,11 @@ def load(path: str | os.PathLike | None = None):
     global _lib, _path
     if _lib is not None and path is None:
         return _lib
+    # PyTorch ships its own HIP runtime under the same soname as ROCm's: it is
+    # loaded first, so that the library's kernels register with the runtime
+    # torch uses (a library loaded before torch pulls in ROCm's copy, and the
+    # process then holds two runtimes, one of which finds no device).
+    import torch  # noqa: F401
     p = Path(path) if path is not None else LIB_PATH
     if not p.exists():
         raise RuntimeError(

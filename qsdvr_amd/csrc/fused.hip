@@ -1250,6 +1250,14 @@ constexpr int tile_channels() {
     return (3 * NC + 5 + 15) / 16 * 16;  // appearance channels padded to 16-wide tiles
 }
 
+// Gradient rows staged in LDS at a time: all of them up to 32 channels; 32
+// (one weight staging's two 16-channel blocks) above, so that the 53-channel
+// adjoint fits 18.3 KB of LDS per wave (8 waves per CU) instead of 27 KB.
+template <int NC>
+constexpr int tile_grad_rows() {
+    return tile_channels<NC>() < 32 ? tile_channels<NC>() : 32;
+}
+
 // At most 256 registers per lane (2 waves per SIMD, 8 per CU, which the
 // 18.25 KiB of LDS per wave also allows: 146 of the CU's 160 KiB): occupancy
 // hides the latency of the per-sample gathers.
@@ -1264,12 +1272,13 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
     const float *__restrict__ tlist, int K, const uint32_t *__restrict__ tile_order) {
     constexpr int C = 3 * NC + 5;
     constexpr int CP = tile_channels<NC>();
+    constexpr int GR = tile_grad_rows<NC>();
     extern __shared__ float4 lds4[];
-    float *gt = reinterpret_cast<float *>(lds4);  // [CP][68]: gradient rows, channel-major
-    float *wt = gt + CP * kGradStride;            // [kCapV][32]: trilinear weights, vertex-major
+    float *gt = reinterpret_cast<float *>(lds4);  // [GR][68]: gradient rows, channel-major
+    float *wt = gt + GR * kGradStride;            // [kCapV][32]: trilinear weights, vertex-major
     float *qb = wt + kCapV * kHalf;               // [7][64]: quadric gradient window
     const int lane = threadIdx.x;
-    for (int i = lane; i < CP * kGradStride + kCapV * kHalf + 7 * kQBoxCells; i += kWave)
+    for (int i = lane; i < GR * kGradStride + kCapV * kHalf + 7 * kQBoxCells; i += kWave)
         gt[i] = 0.0f;
 
     // tiles in scattered order: measured 7% faster here (12% on listed
@@ -1410,8 +1419,11 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
         if (__ballot(app) == 0) continue;  // no appearance or normal gradient this step
         carry_add<6>(qcarry, app, rp, gn7, qb, qbox, R, lane, gq);
         // stage this step's gradient row once, transposed: gt[c][lane]
+        // (more than 32 channels: 32 at a time, below)
+        if constexpr (GR == CP) {
 #pragma unroll
-        for (int c = 0; c < C; ++c) gt[c * kGradStride + lane] = dd[c];
+            for (int c = 0; c < C; ++c) gt[c * kGradStride + lane] = dd[c];
+        }
         bool pend_r = app;  // appearance gradient not yet accumulated
         // Rounds: the tight box of all pending cells when it fits in kCapV
         // vertices (the common case: one round); otherwise a part of it that
@@ -1496,8 +1508,19 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
                 const int nmb = __builtin_amdgcn_readfirstlane((nv + 15) >> 4);
                 // NBG channel blocks share one staging of the weight columns
                 constexpr int NBG = CP / 16 >= 2 ? 2 : 1;
+                static_assert(GR == CP || GR == 16 * NBG, "one gradient-row staging per weight staging");
 #pragma unroll
                 for (int nb0 = 0; nb0 < CP / 16; nb0 += NBG) {
+                    if constexpr (GR < CP) {
+                        // channels 16 nb0 .. 16 nb0 + 31 of the gradient rows (the
+                        // previous ones' operand reads are issued before)
+                        wave_lds_order();
+#pragma unroll
+                        for (int c = 0; c < GR; ++c)
+                            gt[c * kGradStride + lane] =
+                                16 * nb0 + c < C ? dd[16 * nb0 + c < C ? 16 * nb0 + c : 0] : 0.0f;
+                        wave_lds_order();
+                    }
                     floatx4 acc[NBG][4];
 #pragma unroll
                     for (int q = 0; q < NBG; ++q)
@@ -1517,7 +1540,8 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
 #pragma unroll
                         for (int q = 0; q < NBG; ++q) {
                             const float4 *sb = reinterpret_cast<const float4 *>(
-                                gt + (16 * (nb0 + q) + m) * kGradStride + kHalf * h + 8 * g);
+                                gt + (16 * (GR < CP ? q : nb0 + q) + m) * kGradStride +
+                                kHalf * h + 8 * g);
                             b0[q] = sb[0];
                             b1[q] = sb[1];
                         }
@@ -1591,7 +1615,7 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
 
 template <int NC>
 constexpr size_t tile_lds_bytes() {
-    return (size_t)(tile_channels<NC>() * kGradStride + kCapV * kHalf + 7 * kQBoxCells) *
+    return (size_t)(tile_grad_rows<NC>() * kGradStride + kCapV * kHalf + 7 * kQBoxCells) *
            sizeof(float);
 }
 
