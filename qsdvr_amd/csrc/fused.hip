@@ -305,6 +305,68 @@ __device__ __forceinline__ BoxView stage_box(float4 *box, const float *__restric
     return BoxView{box, (rp.ix - lx) + (rp.iy - ly) * ex + (rp.iz - lz) * sxy, ex, sxy};
 }
 
+// 53-channel rows (212 B, only 4-B aligned in the grid) are staged the same
+// way into 56-float LDS rows: 13 16-B chunks (unaligned global loads, aligned
+// LDS stores) and a dword per row, for boxes of up to kBoxV53 vertices
+// (10.5 KB per wave).
+constexpr int kBoxV53 = 48;
+constexpr int kRow53 = 56;  // LDS floats per staged 53-channel row
+
+template <int C>
+__device__ __forceinline__ BoxView stage_box53(float4 *box, const float *__restrict__ data,
+                                               int R, bool need, const CellPoint &rp, int lane,
+                                               int &nv_out) {
+    static_assert(C == 53, "53-channel staging");
+    const int lx = wave_min_s(need ? rp.ix : INT_MAX), hx = wave_max_s(need ? rp.ix : INT_MIN);
+    const int ly = wave_min_s(need ? rp.iy : INT_MAX), hy = wave_max_s(need ? rp.iy : INT_MIN);
+    const int lz = wave_min_s(need ? rp.iz : INT_MAX), hz = wave_max_s(need ? rp.iz : INT_MIN);
+    const int ex = hx - lx + 2, ey = hy - ly + 2, ez = hz - lz + 2;
+    const int sxy = ex * ey, nv = sxy * ez;
+    nv_out = nv;
+    if (nv > kBoxV53) return BoxView{nullptr, 0, 0, 0};
+    const float rsxy = __builtin_amdgcn_rcpf((float)sxy), rex = __builtin_amdgcn_rcpf((float)ex);
+    const int64_t RR = (int64_t)R * R;
+    const float *__restrict__ src = data + ((int64_t)lx + (int64_t)ly * R + lz * RR) * C;
+    float *dst = reinterpret_cast<float *>(box);
+    typedef float f4a __attribute__((ext_vector_type(4), aligned(4)));
+    wave_lds_order();  // the previous step's box reads are issued before these writes
+    for (int i = lane; i < nv * 14; i += kWave) {
+        const int v = i / 14, k = i - 14 * v;
+        const int vz = (int)(((float)v + 0.5f) * rsxy), r2 = v - vz * sxy;
+        const int vy = (int)(((float)r2 + 0.5f) * rex), vx = r2 - vy * ex;
+        const float *row = src + (int64_t)(vx + vy * R + vz * (int)RR) * C;
+        if (k < 13) {
+            const f4a val = *reinterpret_cast<const f4a *>(row + 4 * k);
+            *reinterpret_cast<float4 *>(dst + v * kRow53 + 4 * k) =
+                float4{val.x, val.y, val.z, val.w};
+        } else {
+            dst[v * kRow53 + 52] = row[52];
+        }
+    }
+    wave_lds_order();
+    return BoxView{box, (rp.ix - lx) + (rp.iy - ly) * ex + (rp.iz - lz) * sxy, ex, sxy};
+}
+
+// gather_interp<53> from the staged box (same operation order).
+__device__ __forceinline__ void box_interp53(const BoxView &bx, const float w[8], float d[53]) {
+#pragma unroll
+    for (int c = 0; c < 53; ++c) d[c] = 0.0f;
+    const float *p0 = reinterpret_cast<const float *>(bx.lds) + bx.vb * kRow53;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float *p = p0 + ((k & 1) + ((k >> 1) & 1) * bx.ex + (k >> 2) * bx.sxy) * kRow53;
+#pragma unroll
+        for (int q = 0; q < 13; ++q) {
+            const float4 val = *reinterpret_cast<const float4 *>(p + 4 * q);
+            d[4 * q + 0] += w[k] * val.x;
+            d[4 * q + 1] += w[k] * val.y;
+            d[4 * q + 2] += w[k] * val.z;
+            d[4 * q + 3] += w[k] * val.w;
+        }
+        d[52] += w[k] * p[52];
+    }
+}
+
 struct SampleEval {
     CellPoint rp;     // render sample cell / local point
     f3 normal;
@@ -387,6 +449,9 @@ __device__ __forceinline__ void eval_render_at(const RaySetup &ray, uint32_t j, 
     trilinear_weights(e.rp.local, w);
     if constexpr (C == 32) {
         if (bx.lds != nullptr) box_interp<C>(bx, w, d);
+        else gather_interp<C>(data, e.rp.cell, R, w, d);
+    } else if constexpr (C == 53) {
+        if (bx.lds != nullptr) box_interp53(bx, w, d);
         else gather_interp<C>(data, e.rp.cell, R, w, d);
     } else {
         gather_interp<C>(data, e.rp.cell, R, w, d);
@@ -493,9 +558,10 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
     // Tile waves (rays of one 8x8 pixel tile) stage their appearance rows in
     // LDS; waves of listed rays gather per lane.  The step loop runs
     // wave-uniformly (lanes past their last sample idle) for the staging.
-    const bool stage = C == 32 && ray_pixel == nullptr;
+    const bool stage = (C == 32 || C == 53) && ray_pixel == nullptr;
     const int lane = threadIdx.x % kWave;
-    float4 *box = fwd_lds + (threadIdx.x / kWave) * (kBoxV * (C / 4 + 1));
+    float4 *box = fwd_lds + (threadIdx.x / kWave) * (C == 32 ? kBoxV * (C / 4 + 1)
+                                                             : kBoxV53 * kRow53 / 4);
     const float half_reso = (float)R / 2.0f, inv_reso = 1.0f / (float)R;
     f3 col{0.0f, 0.0f, 0.0f};
     float dep = 0.0f;
@@ -562,6 +628,11 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
                     int nv = 0;
                     bx = stage_box<C>(box, data, R, need, rp, lane, nv);
                     if constexpr (COST) cost += (uint32_t)nv + (nv > kBoxV ? 4u * (uint32_t)nv : 0u);
+                }
+            } else if constexpr (C == 53) {
+                if (stage) {
+                    int nv = 0;
+                    bx = stage_box53<C>(box, data, R, need, rp, lane, nv);
                 }
             }
             if (need) {
@@ -1621,7 +1692,8 @@ constexpr size_t tile_lds_bytes() {
 
 template <int NC>
 constexpr size_t fwd_lds_bytes() {
-    return 3 * NC + 5 == 32 ? (size_t)kWavesPerBlock * kBoxV * ((3 * NC + 5) / 4 + 1) * 16 : 0;
+    return 3 * NC + 5 == 32 ? (size_t)kWavesPerBlock * kBoxV * ((3 * NC + 5) / 4 + 1) * 16
+                            : (size_t)kWavesPerBlock * kBoxV53 * kRow53 * 4;
 }
 
 static int64_t fused_waves(uint32_t W, uint32_t H, int n_cams, const int32_t *ray_pixel,
