@@ -320,6 +320,10 @@ __device__ __forceinline__ BoxView stage_box53(float4 *box, const float *__restr
     const int lx = wave_min_s(need ? rp.ix : INT_MAX), hx = wave_max_s(need ? rp.ix : INT_MIN);
     const int ly = wave_min_s(need ? rp.iy : INT_MAX), hy = wave_max_s(need ? rp.iy : INT_MIN);
     const int lz = wave_min_s(need ? rp.iz : INT_MAX), hz = wave_max_s(need ? rp.iz : INT_MIN);
+    if (hx < lx) {  // no lane needs rows
+        nv_out = 0;
+        return BoxView{nullptr, 0, 0, 0};
+    }
     const int ex = hx - lx + 2, ey = hy - ly + 2, ez = hz - lz + 2;
     const int sxy = ex * ey, nv = sxy * ez;
     nv_out = nv;
