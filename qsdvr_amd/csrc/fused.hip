@@ -1480,13 +1480,19 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
         if (act_r) {
             carry = g_sdf1;
             cp0 = cp1;
-            if constexpr (QAHEAD) {
-                qs0 = qs1;
-                if ((uint32_t)(j + 2) <= ray.renders) {  // sdf sample j + 2, for the next step
-                    cp1 = ray_sdf_point_t(ray, j + 2, tw.t1(ray, j + 1), interval, half_reso, R);
-                    qs1 = load_quadric(qgrid, cp1.cell);
-                }
-            }
+            if constexpr (QAHEAD) qs0 = qs1;
+        }
+        if constexpr (QAHEAD) {
+            // sdf sample j + 2, for the next step, requested by every lane (past
+            // its ray's end a lane loads a clamped cell's quadric and never uses
+            // it): the load then writes the loop-carried registers themselves,
+            // and the step's end does not wait for it (a lane-masked load is
+            // merged into them with copies that wait for every outstanding
+            // load and atomic of the step)
+            const CellPoint cn =
+                ray_sdf_point_t(ray, j + 2, tw.t1(ray, j + 1), interval, half_reso, R);
+            if (act_r && (uint32_t)(j + 2) <= ray.renders) cp1 = cn;
+            qs1 = load_quadric(qgrid, cn.cell);
         }
 
         // quadric gradients of the sdf sample and of the render sample
