@@ -396,19 +396,30 @@ __device__ __forceinline__ CellPoint point_at(const RaySetup &ray, float t, floa
 template <bool AHEAD>
 struct TWindow {
     float a = 0.0f, b = 0.0f, c = 0.0f;
-    __device__ __forceinline__ void start(const RaySetup &ray) {
-        if (AHEAD && ray.tl != nullptr) {
-            a = ray.tl[0];
-            b = ray.tl[1];
-            c = ray.renders >= 2 ? ray.tl[2] : 0.0f;
+    // AHEAD: the list the window reads, and its last index.  A lane without
+    // a list (an idle lane) reads entry 0 of `spare` instead, and every lane
+    // advances every step (past the ray's end the last entry repeats): the
+    // loads are then not lane-masked and write the window's registers
+    // themselves, where a masked load is merged into them with copies that
+    // wait for every outstanding load and atomic.
+    const float *src = nullptr;
+    uint32_t last = 0;
+    // spare: any readable list entry (the kernel's list argument)
+    __device__ __forceinline__ void start(const RaySetup &ray, const float *spare) {
+        if (AHEAD) {
+            src = ray.tl != nullptr ? ray.tl : spare;
+            last = ray.tl != nullptr ? ray.renders : 0u;
+            a = src[0];
+            b = src[min(1u, last)];
+            c = src[min(2u, last)];
         }
     }
     // step j done: t_{j+1}, t_{j+2} move down and t_{j+3} is requested
-    __device__ __forceinline__ void advance(const RaySetup &ray, int j) {
-        if (AHEAD && ray.tl != nullptr) {
+    __device__ __forceinline__ void advance(int j) {
+        if (AHEAD) {
             a = b;
             b = c;
-            c = (uint32_t)(j + 3) <= ray.renders ? ray.tl[j + 3] : 0.0f;
+            c = src[min((uint32_t)(j + 3), last)];
         }
     }
     // t_j and t_{j+1} at step j
@@ -572,7 +583,7 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
     const int steps = wave_max(live ? (int)ray.renders : 0);
     CellPoint cp;
     TWindow<LISTED && C == 32> tw;
-    if (live) tw.start(ray);
+    tw.start(ray, tlist);
     float e0 = live ? expf(-s * sdf_at_t(ray, 0, tw.t0(ray, 0), interval, half_reso, R, inv_reso,
                                          qgrid, cp))
                     : 1.0f;
@@ -601,16 +612,20 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
         if (act) {
             if constexpr (QAHEAD) {
                 e1 = expf(-s * quadric_sdf(qn, cp.local, inv_reso));
-                if ((uint32_t)(j + 2) <= ray.renders) {  // sdf sample j + 2, for the next step
-                    cp = ray_sdf_point_t(ray, j + 2, tw.t2(ray, j), interval, half_reso, R);
-                    qn = load_quadric(qgrid, cp.cell);
-                }
             } else {
                 e1 = expf(-s * sdf_at_t(ray, j + 1, tw.t1(ray, j), interval, half_reso, R,
                                         inv_reso, qgrid, cp));
             }
             a = fmaxf((e1 - e0) * fast_rcp(1.0f + e1), 0.0f);
             wt = a * T;
+        }
+        if constexpr (QAHEAD) {
+            // sdf sample j + 2, for the next step, requested by every lane (past
+            // its ray's end a lane loads a clamped cell's quadric and never uses
+            // it), so that the load is not lane-masked
+            const CellPoint cn = ray_sdf_point_t(ray, j + 2, tw.t2(ray, j), interval, half_reso, R);
+            if (act && (uint32_t)(j + 2) <= ray.renders) cp = cn;
+            qn = load_quadric(qgrid, cn.cell);
         }
         // A sample of weight exactly zero (alpha clamped to 0 where the SDF
         // rises along the ray, or T underflowed) adds nothing: its
@@ -662,8 +677,8 @@ __global__ void __launch_bounds__(kFusedBlock, NC == 9 ? 4 : 1) fused_fwd_kernel
             }
             T = T * (1.0f - a);
             e0 = e1;
-            tw.advance(ray, j);
         }
+        tw.advance(j);
     }
     if constexpr (COST) {
         if (ray_pixel == nullptr && lane == 0) {
@@ -1389,8 +1404,8 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
     constexpr bool QAHEAD = C == 32;
     Quadric qs0, qs1;
     TWindow<LISTED && C == 32> tw;
+    tw.start(ray, tlist);
     if (live) {
-        tw.start(ray);
         cp0 = ray_sdf_point_t(ray, 0, tw.t0(ray, 0), interval, half_reso, R);
         qs0 = load_quadric(qgrid, cp0.cell);
         if constexpr (QAHEAD) {
@@ -1471,8 +1486,8 @@ __global__ void __launch_bounds__(kWave, 2) fused_bwd_tile_kernel(
             }
             T = T * (1.0f - a);
             e0 = e1;
-            tw.advance(ray, j);
         }
+        tw.advance(j);
         const CellPoint sc = cp0;  // sdf sample j
         if (act_s)
             quadric_sdf_grad(QAHEAD ? qs0 : load_quadric(qgrid, sc.cell), sc.local, g_sdf0,
