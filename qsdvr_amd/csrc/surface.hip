@@ -175,21 +175,26 @@ __global__ void __launch_bounds__(256) sample_sdf_fwd_kernel(float *__restrict__
                                                              int64_t n, int64_t cells,
                                                              float inv_reso) {
     const int64_t base = xcd_block() * blockDim.x * kSampleSPL + threadIdx.x;
+    // every lane loads its indices (a lane past the end reads the last one
+    // and marks it invalid afterwards): a lane-masked load would be merged
+    // with a wait before the next one is issued
     int64_t c[kSampleSPL];
 #pragma unroll
     for (int u = 0; u < kSampleSPL; ++u) {
         const int64_t i = base + (int64_t)u * blockDim.x;
-        c[u] = i < n ? (int64_t)idx[i] : -1;
+        c[u] = (int64_t)idx[i < n ? i : n - 1];
     }
+    // the quadrics and points likewise (an invalid sample reads cell 0 and the
+    // last point, and gets NaN below): a masked block also pulls the first
+    // arithmetic on its loads in with a wait
     Quadric q[kSampleSPL];
     f3 p[kSampleSPL];
 #pragma unroll
     for (int u = 0; u < kSampleSPL; ++u) {
         const int64_t i = base + (int64_t)u * blockDim.x;
-        if (c[u] >= 0 && c[u] < cells) {
-            q[u] = load_quadric(grid, c[u]);
-            p[u] = load3(pts, i);
-        }
+        if (i >= n) c[u] = -1;
+        q[u] = load_quadric(grid, c[u] >= 0 && c[u] < cells ? c[u] : 0);
+        p[u] = load3(pts, i < n ? i : n - 1);
     }
 #pragma unroll
     for (int u = 0; u < kSampleSPL; ++u) {
@@ -223,21 +228,26 @@ __global__ void __launch_bounds__(256) sample_normal_fwd_kernel(float *__restric
                                                                 const int32_t *__restrict__ idx,
                                                                 int64_t n, int64_t cells) {
     const int64_t base = xcd_block() * blockDim.x * kSampleSPL + threadIdx.x;
+    // every lane loads its indices (a lane past the end reads the last one
+    // and marks it invalid afterwards): a lane-masked load would be merged
+    // with a wait before the next one is issued
     int64_t c[kSampleSPL];
 #pragma unroll
     for (int u = 0; u < kSampleSPL; ++u) {
         const int64_t i = base + (int64_t)u * blockDim.x;
-        c[u] = i < n ? (int64_t)idx[i] : -1;
+        c[u] = (int64_t)idx[i < n ? i : n - 1];
     }
+    // the quadrics and points likewise (an invalid sample reads cell 0 and the
+    // last point, and gets NaN below): a masked block also pulls the first
+    // arithmetic on its loads in with a wait
     Quadric q[kSampleSPL];
     f3 p[kSampleSPL];
 #pragma unroll
     for (int u = 0; u < kSampleSPL; ++u) {
         const int64_t i = base + (int64_t)u * blockDim.x;
-        if (c[u] >= 0 && c[u] < cells) {
-            q[u] = load_quadric(grid, c[u]);
-            p[u] = load3(pts, i);
-        }
+        if (i >= n) c[u] = -1;
+        q[u] = load_quadric(grid, c[u] >= 0 && c[u] < cells ? c[u] : 0);
+        p[u] = load3(pts, i < n ? i : n - 1);
     }
 #pragma unroll
     for (int u = 0; u < kSampleSPL; ++u) {
