@@ -256,9 +256,20 @@ __global__ void __launch_bounds__(256) quadric_seg_kernel(const uint32_t *__rest
     const int64_t p0 = w * kWave;
     if (p0 >= n) return;  // wave-uniform
     const int64_t p = p0 + lane;
-    const uint32_t k = p < n ? key[p] : invalid;
+    // Every lane requests its key and the wave's two neighbouring keys at
+    // once (positions clamped to the records), then its record and quadric
+    // (cell 0 for an invalid key, whose sums are dropped): two memory round
+    // trips, where lane-masked requests are each waited for in turn.
+    const uint32_t kraw = key[p < n ? p : n - 1];
+    const uint32_t kleft = key[p0 > 0 ? p0 - 1 : 0];
+    const uint32_t kright = key[p0 + kWave < n ? p0 + kWave : n - 1];
+    const uint32_t k = p < n ? kraw : invalid;
     float v[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (k != invalid) src.add(p, keys.cell(k), v);
+    src.add(p < n ? p : n - 1, k != invalid ? keys.cell(k) : 0, v);
+    if (k == invalid) {
+#pragma unroll
+        for (int j = 0; j < 7; ++j) v[j] = 0.0f;
+    }
     const uint32_t kp = __shfl_up(k, 1, kWave), kn = __shfl_down(k, 1, kWave);
     const bool start = lane == 0 || kp != k;
     const bool last = lane == kWave - 1 || kn != k;
@@ -277,8 +288,8 @@ __global__ void __launch_bounds__(256) quadric_seg_kernel(const uint32_t *__rest
         }
     }
     const uint32_t k0 = __shfl(k, 0, kWave), k63 = __shfl(k, kWave - 1, kWave);
-    const bool cont_in = p0 > 0 && key[p0 - 1] == k0;
-    const bool cont_out = p0 + kWave < n && key[p0 + kWave] == k63;
+    const bool cont_in = p0 > 0 && kleft == k0;
+    const bool cont_out = p0 + kWave < n && kright == k63;
     if (last && k != invalid) {
         // the segment holds lane 0 when no segment starts in lanes 1..lane
         const bool has0 = (starts & ((2ull << lane) - 2ull)) == 0;
