@@ -275,9 +275,18 @@ __global__ void __launch_bounds__(256) aggregate_fwd_kernel(
         for (uint32_t c0 = 0; c0 < rc; c0 += kWave) {
             const uint32_t i = c0 + lane;
             const bool valid = i < rc;
-            const float e0 = i <= rc ? expf(-s * sdf[(int64_t)so + i]) : 0.0f;
+            // every lane requests its sdf values and colour at once, at positions
+            // clamped to the ray (a lane past the end computes values no valid
+            // lane reads): lane-masked requests are each waited for in turn
+            // (sdf_j, sdf_{j+1}) as one 8-B request at j = min(i, rc - 1): a lane
+            // at i = rc takes sdf_rc from its second half
+            typedef float f2a __attribute__((ext_vector_type(2), aligned(4)));
+            const f2a sp =
+                *reinterpret_cast<const f2a *>(sdf + (int64_t)so + (valid ? i : rc - 1));
+            const f3 c = load3(rgb, (int64_t)ro + (valid ? i : rc - 1));
+            const float e0 = i <= rc ? expf(-s * (valid ? sp.x : sp.y)) : 0.0f;
             float e1 = __shfl_down(e0, 1, kWave);
-            if (lane == kWave - 1 && i + 1 <= rc) e1 = expf(-s * sdf[(int64_t)so + i + 1]);
+            if (lane == kWave - 1 && i + 1 <= rc) e1 = expf(-s * sp.y);
             const float a = valid ? fminf(fmaxf((e1 - e0) / (1.0f + e1), 0.0f), 10.0f) : 0.0f;
             float p = 1.0f - a;
 #pragma unroll
@@ -290,7 +299,6 @@ __global__ void __launch_bounds__(256) aggregate_fwd_kernel(
             if (valid) {
                 alpha[(int64_t)ro + i] = a;
                 T_out[(int64_t)ro + i] = Ti;
-                const f3 c = load3(rgb, (int64_t)ro + i);
                 col.x += c.x * a * Ti;
                 col.y += c.y * a * Ti;
                 col.z += c.z * a * Ti;
