@@ -527,14 +527,14 @@ __global__ void __launch_bounds__(256) sampler_kernel(const float *__restrict__ 
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int64_t r = (int64_t)blockIdx.x * 4 + wv;
     if (r >= n_rays) return;  // whole wavefront
+    // the ray's count and its logits requested together (one round trip): a
+    // ray with no segment (count < 2) runs the same code on its unused logits
+    // and writes all its samples at the origin
     const uint32_t count = __float_as_uint(ray_state[2 * r].w);
-    float *out = tlist + r * K;
-    if (count < 2) {  // no segment: all samples at the origin
-        for (int k = lane; k < K; k += kWave) out[k] = 0.0f;
-        return;
-    }
-    const float L = (float)(count - 1) * interval;
     const float z = logits[r * kBins + lane];
+    float *out = tlist + r * K;
+    const bool empty = count < 2;
+    const float L = empty ? 0.0f : (float)(count - 1) * interval;
     const float d = z > 15.0f ? z : log1pf(expf(z));
     const float S = wave_scan_incl(d, lane);
     const float w1 = expf(-(S - d)) * -expm1f(-d);
@@ -556,7 +556,7 @@ __global__ void __launch_bounds__(256) sampler_kernel(const float *__restrict__ 
         for (int step = kBins / 2; step > 0; step >>= 1)
             if (lo + step < kBins && s_lo[wv][lo + step] <= u) lo += step;
         const float frac = fminf(fmaxf((u - s_lo[wv][lo]) / s_pdf[wv][lo], 0.0f), 1.0f);
-        out[1 + k] = ((float)lo + frac) * (1.0f / kBins) * L;
+        out[1 + k] = empty ? 0.0f : ((float)lo + frac) * (1.0f / kBins) * L;
     }
     if (lane == 0) {
         out[0] = 0.0f;
